@@ -1,0 +1,260 @@
+#!/usr/bin/env python
+"""Krylov time-steps/s of the N=26 triangular-lattice workload on one MI355X.
+
+Workload: 26 atoms on a triangular lattice (5.5 um, rows of 7/6/7/6),
+Omega = 2 pi rad/us with sin^2 rise/fall over 200 ns, delta ramp -20 -> 15
+rad/us over 3000 ns, dt = 10 ns (300 Krylov steps), Lanczos tolerance 1e-10,
+initial state |g...g>.  A "step" is one Krylov time step exp(-i dt H) psi.
+
+Timed: ``--steps K`` consecutive steps from t = 0 (K = 300 is the whole
+sequence) after ``--warmup W`` untimed steps, bracketed by device syncs,
+first in complex128 (``value``), then in complex64 with FP64 Lanczos scalars
+(reported with its fidelity against the complex128 run).
+
+Also measured: the fused Lanczos H.psi (all passes, with its reductions)
+timed with HIP events on the stream it runs on, against HBM peak; and the
+CPU baseline (the oracle, a restatement of the reference's torch-CPU
+algorithm) on a bounded sample of the same workload.
+
+Output: one JSON line on stdout.  Progress goes to stderr.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Krylov steps/s + H·ψ HBM GB/s at N=26 (c64 vs c128 at stated fidelity)"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_steps(seq, psi, k0, count, dtype, tol, stats):
+    import torch
+
+    from emulators_amd.emu_sv.time_evolution import EvolveStateVector
+
+    stepper = EvolveStateVector()
+    u = seq.interaction_matrix(0.0)[0]
+    for k in range(k0, k0 + count):
+        dt = (seq.target_times[k + 1] - seq.target_times[k]) * 1e-3
+        psi, _ = stepper.apply(dt, seq.omega[k], seq.delta[k], seq.phi[k], u, psi, tol, [])
+        if (k + 1) % 50 == 0:
+            log(f"{dtype}: step {k + 1}, matvecs so far {stats.matvecs}")
+    return psi
+
+
+def timed_run(seq, dtype, steps, warmup, tol):
+    import torch
+
+    from emulators_amd.emu_base.math.krylov_exp import KrylovStats
+
+    n = seq.omega.shape[1]
+    psi0 = torch.zeros(2**n, dtype=dtype, device="cuda")
+    psi0[0] = 1.0
+    if warmup:
+        run_steps(seq, psi0.clone(), 0, warmup, dtype, tol, KrylovStats)
+    torch.cuda.synchronize()
+    KrylovStats.reset()
+    t0 = time.perf_counter()
+    psi = run_steps(seq, psi0.clone(), 0, steps, dtype, tol, KrylovStats)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return psi, wall, KrylovStats.matvecs
+
+
+def time_hpsi(seq, dtype, reps, step):
+    """Average duration of one fused Lanczos H.psi (all passes + reductions)."""
+    import torch
+
+    from emulators_amd import _native as native
+    from emulators_amd.emu_sv import RydbergHamiltonian
+
+    n = seq.omega.shape[1]
+    ham = RydbergHamiltonian(seq.omega[step], seq.delta[step], seq.phi[step],
+                             seq.interaction_matrix(0.0)[0], device="cuda")
+    params = ham.device_params(torch.device("cuda"))
+    g = torch.Generator(device="cuda").manual_seed(0)
+    v = torch.randn(2**n, dtype=dtype, device="cuda", generator=g)
+    prev = torch.randn(2**n, dtype=dtype, device="cuda", generator=g)
+    w = torch.empty_like(v)
+    for _ in range(3):
+        native.hpsi(n, params, -0.01j, v, 1.0, w, prev=prev, prev_scale=1.0, dots=True,
+                    complex_coef=ham.complex)
+    stream = torch.cuda.current_stream()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(reps):
+        native.hpsi(n, params, -0.01j, v, 1.0, w, prev=prev, prev_scale=1.0, dots=True,
+                    complex_coef=ham.complex)
+    stop.record(stream)
+    stop.synchronize()
+    ms = start.elapsed_time(stop) / reps
+    bpe = 16 if dtype == torch.complex128 else 8
+    alg_bytes = 3 * (2**n) * bpe  # read v_j, read v_{j-1}, write w
+    passes = native.plan_passes(n, dtype)
+    del v, prev, w
+    return ms, alg_bytes, passes
+
+
+def cpu_baseline(seq, mean_iters, step, sample_iters):
+    """Oracle (torch CPU, complex128) on the N=26 Hamiltonian of one step:
+    the diagonal build plus ``sample_iters`` Lanczos iterations, scaled to
+    one Krylov step of ``mean_iters`` iterations."""
+    import torch
+
+    from oracle import emu_sv_oracle as orc
+
+    threads = min(os.cpu_count() or 1, 16)
+    torch.set_num_threads(threads)
+    n = seq.omega.shape[1]
+    dt = 0.01
+    t0 = time.perf_counter()
+    ham = orc.OracleHamiltonian(seq.omega[step], seq.delta[step], seq.phi[step],
+                                seq.interaction_matrix(0.0)[0])
+    t_diag = time.perf_counter() - t0
+    v = torch.zeros(2**n, dtype=torch.complex128)
+    v[0] = 1.0
+    prev = None
+    t0 = time.perf_counter()
+    for _ in range(sample_iters):
+        w = -1j * dt * (ham * v)
+        w.norm()
+        if prev is not None:
+            a = torch.vdot(prev, w)
+            w -= a * prev
+        b = torch.vdot(v, w)
+        w -= b * v
+        nrm = w.norm()
+        w /= nrm
+        prev, v = v, w
+    t_iter = (time.perf_counter() - t0) / sample_iters
+    t_step = t_diag + mean_iters * t_iter
+    return {
+        "value": 1.0 / t_step,
+        "unit": "Krylov steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle (torch-CPU complex128 restatement of the reference) on the N=26 "
+                   f"Hamiltonian of step {step}: diagonal build {t_diag:.2f} s + "
+                   f"{sample_iters} Lanczos iterations at {t_iter:.2f} s each, scaled to "
+                   f"{mean_iters:.1f} iterations per step (the GPU run's mean)"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--hpsi-reps", type=int, default=20)
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c64", action="store_true")
+    ap.add_argument("--duration", type=int, default=3000, help="pulse duration in ns")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the single-GPU N=26 state fits in one MI355X)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from emulators_amd import _native as native
+    from emulators_amd import workloads
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 2
+    tol = 1e-10
+    seq = workloads.triangular26(duration=args.duration)
+    nsteps = seq.omega.shape[0]
+    steps = min(args.steps, nsteps)
+    n = seq.omega.shape[1]
+    log(f"workload triangular26: N={n}, {nsteps} steps; timing {steps} after {args.warmup} warmup")
+
+    psi128, wall128, mv128 = timed_run(seq, torch.complex128, steps, args.warmup, tol)
+    log(f"c128: {steps} steps in {wall128:.2f} s, {mv128} matvecs")
+    result64 = None
+    if not args.no_c64:
+        psi64, wall64, mv64 = timed_run(seq, torch.complex64, steps, args.warmup, tol)
+        log(f"c64: {steps} steps in {wall64:.2f} s, {mv64} matvecs")
+        a = psi128
+        b = native.convert(n, psi64, torch.empty_like(psi128))
+        fid = float(torch.abs(torch.vdot(a, b)) ** 2 / (torch.vdot(a, a).real * torch.vdot(b, b).real))
+        occ_a = native.correlation(n, a).diagonal()
+        occ_b = native.correlation(n, b).diagonal()
+        result64 = {"value": steps / wall64, "ms_per_step": 1e3 * wall64 / steps,
+                    "speedup_vs_c128": wall128 / wall64, "fidelity_vs_c128": fid,
+                    "fidelity_deficit": 1.0 - fid,
+                    "max_occupation_error": float(torch.max(torch.abs(occ_a - occ_b))),
+                    "mean_lanczos_iterations": mv64 / steps}
+        del psi64, b
+    del psi128
+    torch.cuda.empty_cache()
+
+    mid = min(nsteps // 2, nsteps - 1)
+    ms128, bytes128, passes128 = time_hpsi(seq, torch.complex128, args.hpsi_reps, mid)
+    gbs128 = bytes128 / (ms128 * 1e-3) / 1e9
+    hpsi = {"c128": {"ms": ms128, "alg_bytes": bytes128, "GB/s": gbs128, "passes": passes128}}
+    if not args.no_c64:
+        ms64, bytes64, passes64 = time_hpsi(seq, torch.complex64, args.hpsi_reps, mid)
+        hpsi["c64"] = {"ms": ms64, "alg_bytes": bytes64, "GB/s": bytes64 / (ms64 * 1e-3) / 1e9,
+                       "passes": passes64}
+    log(f"H.psi c128 {ms128:.3f} ms ({gbs128:.0f} GB/s algorithmic)")
+
+    mean_iters = mv128 / steps
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("CPU baseline (oracle) ...")
+        cpu = cpu_baseline(seq, mean_iters, mid, args.cpu_iters)
+        log(f"CPU baseline {cpu['value']:.4f} steps/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": steps / wall128,
+        "unit": "Krylov steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall128 / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic",
+        "config": {
+            "workload": "triangular26: N=26 triangular lattice 5.5 um (rows 7/6/7/6), Omega 2pi "
+                        "rad/us sin^2 rise/fall 200 ns, delta -20->15 rad/us, "
+                        f"{args.duration} ns at dt=10 ns, |g..g>, Krylov tolerance 1e-10",
+            "n_qubits": n,
+            "time_steps": steps,
+            "mean_lanczos_iterations": mean_iters,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": gbs128,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": gbs128 / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": f"fused Lanczos H.psi, complex128, {passes128} LDS-tiled passes "
+                      "(hpsi_pass_kernel) + reduction; algorithmic bytes 3*D*16 per call",
+        },
+        "cpu_baseline": cpu,
+        "c64": result64,
+        "hpsi": hpsi,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,258 @@
+"""ctypes binding of ``libemu_sv_hip.so`` (the C ABI in include/emu_sv_hip.h).
+
+The library is loaded from ``emulators_amd/_lib``.  When it is missing the
+import of this module raises: there is no CPU or pure-torch substitute for
+the state-vector kernels.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libemu_sv_hip.so"
+
+EMU_SV_OK = 0
+EMU_SV_EINVAL = -1
+EMU_SV_EHIP = -2
+EMU_SV_ENOTCONVERGED = -3
+EMU_SV_C128 = 0
+EMU_SV_C64 = 1
+EMU_SV_FLAG_COMPLEX = 1
+EMU_SV_MAX_QUBITS = 32
+EMU_SV_HAM_DOUBLES = 1128
+
+# Every symbol declared in include/emu_sv_hip.h, with (restype, argtypes).
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+_dp = ctypes.POINTER(ctypes.c_double)
+SIGNATURES = {
+    "emu_sv_last_error": (ctypes.c_char_p, []),
+    "emu_sv_version": (ctypes.c_char_p, []),
+    "emu_sv_pack_hamiltonian": (_i, [_i, _dp, _dp, _dp, _dp, _dp]),
+    "emu_sv_plan_passes": (_i, [_i, _i, _i, _i]),
+    "emu_sv_workspace_bytes": (ctypes.c_size_t, [_i, _i]),
+    "emu_sv_hpsi": (_i, [_i, _i, _vp, _d, _d, _vp, _d, _vp, _d, _vp, _vp, _vp, _i, _i, _i, _vp]),
+    "emu_sv_lanczos_update": (_i, [_i, _i, _vp, _vp, _d, _d, _d, _vp, _d, _d, _d, _vp, _vp, _vp]),
+    "emu_sv_lincomb": (_i, [_i, _i, _i, ctypes.POINTER(_vp), _dp, _vp, _vp]),
+    "emu_sv_dot": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp]),
+    "emu_sv_scale": (_i, [_i, _i, _vp, _d, _vp]),
+    "emu_sv_convert": (_i, [_i, _i, _vp, _i, _vp, _vp]),
+    "emu_sv_correlation": (_i, [_i, _i, _vp, _vp, _vp, _vp]),
+    "emu_sv_expm": (_i, [_i, _dp, _dp]),
+}
+
+
+def _load() -> ctypes.CDLL:
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m emulators_amd.build` "
+            "(the emu_sv engine has no CPU fallback)"
+        )
+    lib = ctypes.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | ctypes.RTLD_GLOBAL)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+LIB = _load()
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str) -> None:
+    """Map a C ABI status to the reference's exception types."""
+    if rc == EMU_SV_OK:
+        return
+    msg = LIB.emu_sv_last_error().decode(errors="replace")
+    if rc == EMU_SV_EINVAL:
+        raise ValueError(f"{what}: {msg}")
+    if rc == EMU_SV_ENOTCONVERGED:
+        raise RecursionError(msg)
+    raise NativeError(f"{what}: {msg}")
+
+
+def version() -> str:
+    return LIB.emu_sv_version().decode()
+
+
+def dtype_code(dtype: torch.dtype) -> int:
+    if dtype == torch.complex128:
+        return EMU_SV_C128
+    if dtype == torch.complex64:
+        return EMU_SV_C64
+    raise ValueError(f"state vectors must be complex64 or complex128, got {dtype}")
+
+
+def _ptr(t: torch.Tensor | None):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(device: torch.device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def nqubits_of(vec: torch.Tensor) -> int:
+    d = vec.numel()
+    n = d.bit_length() - 1
+    if d < 2 or (1 << n) != d:
+        raise ValueError(f"state length must be a power of two >= 2, got {d}")
+    return n
+
+
+def _require_device_vec(vec: torch.Tensor, what: str) -> None:
+    if not vec.is_cuda:
+        raise ValueError(f"{what}: the emu_sv engine needs device (GPU) tensors, got {vec.device}")
+    if not vec.is_contiguous():
+        raise ValueError(f"{what}: tensor must be contiguous")
+
+
+def pack_hamiltonian(omega, delta, phi, interaction) -> tuple[np.ndarray, bool]:
+    """Host packing of the parameter block (real float64 per qubit)."""
+    om = np.ascontiguousarray(np.asarray(omega, dtype=np.float64))
+    de = np.ascontiguousarray(np.asarray(delta, dtype=np.float64))
+    ph = np.ascontiguousarray(np.asarray(phi, dtype=np.float64))
+    um = np.ascontiguousarray(np.asarray(interaction, dtype=np.float64))
+    n = om.shape[0]
+    if de.shape != (n,) or ph.shape != (n,) or um.shape != (n, n):
+        raise ValueError("pack_hamiltonian: inconsistent parameter shapes")
+    block = np.zeros(EMU_SV_HAM_DOUBLES, dtype=np.float64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(LIB.emu_sv_pack_hamiltonian(n, om.ctypes.data_as(dp), de.ctypes.data_as(dp),
+                                      ph.ctypes.data_as(dp), um.ctypes.data_as(dp),
+                                      block.ctypes.data_as(dp)), "pack_hamiltonian")
+    return block, bool(np.any(ph != 0.0))
+
+
+def plan_passes(n: int, dtype: torch.dtype, tile_log: int = 0, min_contig: int = 0) -> int:
+    rc = LIB.emu_sv_plan_passes(n, dtype_code(dtype), tile_log, min_contig)
+    if rc < 0:
+        check(rc, "plan_passes")
+    return rc
+
+
+class Workspace:
+    """Per-device scratch: partial sums plus small result slots."""
+
+    _cache: dict = {}
+
+    def __init__(self, device: torch.device, n: int):
+        nbytes = LIB.emu_sv_workspace_bytes(n, 0)
+        self.scratch = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        self.results = torch.zeros(64, dtype=torch.float64, device=device)
+        self.n = n
+
+    @classmethod
+    def get(cls, device: torch.device, n: int) -> "Workspace":
+        key = (device.index if device.index is not None else torch.cuda.current_device())
+        ws = cls._cache.get(key)
+        if ws is None or ws.n < n:
+            ws = Workspace(device, max(n, 1))
+            cls._cache[key] = ws
+        return ws
+
+
+def hpsi(n: int, ham_dev: torch.Tensor, sigma: complex, v: torch.Tensor, v_scale: float,
+         w: torch.Tensor, prev: torch.Tensor | None = None, prev_scale: float = 1.0,
+         dots: bool = False, complex_coef: bool = True, tile_log: int = 0,
+         min_contig: int = 0) -> torch.Tensor | None:
+    """w = sigma * H (v_scale v); optionally returns the 7 fused dots (device)."""
+    _require_device_vec(v, "hpsi")
+    _require_device_vec(w, "hpsi")
+    if v.dtype != w.dtype or v.numel() != w.numel() or v.numel() != (1 << n):
+        raise ValueError("hpsi: v and w must be 2**n-element vectors of one dtype")
+    if prev is not None and (prev.dtype != v.dtype or prev.numel() != v.numel()):
+        raise ValueError("hpsi: prev must match v")
+    ws = Workspace.get(v.device, n)
+    out = ws.results[0:7] if dots else None
+    flags = EMU_SV_FLAG_COMPLEX if complex_coef else 0
+    check(LIB.emu_sv_hpsi(n, dtype_code(v.dtype), _ptr(ham_dev), float(sigma.real),
+                          float(sigma.imag), _ptr(v), float(v_scale), _ptr(prev),
+                          float(prev_scale), _ptr(w), _ptr(out), _ptr(ws.scratch), tile_log,
+                          min_contig, flags, _stream(v.device)), "hpsi")
+    return out
+
+
+def lanczos_update(n: int, w: torch.Tensor, pa: torch.Tensor | None, pa_scale: float, a: complex,
+                   pb: torch.Tensor | None, pb_scale: float, b: complex) -> torch.Tensor:
+    """w -= a pa_scale pa + b pb_scale pb; returns a device slot holding ||w||^2."""
+    _require_device_vec(w, "lanczos_update")
+    ws = Workspace.get(w.device, n)
+    out = ws.results[8:9]
+    check(LIB.emu_sv_lanczos_update(n, dtype_code(w.dtype), _ptr(w), _ptr(pa), float(pa_scale),
+                                    float(a.real), float(a.imag), _ptr(pb), float(pb_scale),
+                                    float(b.real), float(b.imag), _ptr(out), _ptr(ws.scratch),
+                                    _stream(w.device)), "lanczos_update")
+    return out
+
+
+def lincomb(n: int, vecs: list[torch.Tensor], coef: np.ndarray, out: torch.Tensor) -> torch.Tensor:
+    _require_device_vec(out, "lincomb")
+    count = len(vecs)
+    if count != len(coef):
+        raise ValueError("lincomb: one coefficient per vector")
+    for v in vecs:
+        _require_device_vec(v, "lincomb")
+        if v.dtype != out.dtype or v.numel() != out.numel() or v.numel() != (1 << n):
+            raise ValueError("lincomb: vectors must match the output (2**n elements, one dtype)")
+    arr = (ctypes.c_void_p * count)(*[v.data_ptr() for v in vecs])
+    c = np.ascontiguousarray(np.asarray(coef, dtype=np.complex128)).view(np.float64)
+    check(LIB.emu_sv_lincomb(n, dtype_code(out.dtype), count, arr,
+                             c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _ptr(out),
+                             _stream(out.device)), "lincomb")
+    return out
+
+
+def dot(n: int, a: torch.Tensor, b: torch.Tensor | None = None) -> torch.Tensor:
+    """Device slot [||a||^2, Re<a|b>, Im<a|b>]."""
+    _require_device_vec(a, "dot")
+    ws = Workspace.get(a.device, n)
+    out = ws.results[16:19]
+    check(LIB.emu_sv_dot(n, dtype_code(a.dtype), _ptr(a), _ptr(b), _ptr(out), _ptr(ws.scratch),
+                         _stream(a.device)), "dot")
+    return out
+
+
+def scale(n: int, v: torch.Tensor, s: float) -> None:
+    _require_device_vec(v, "scale")
+    check(LIB.emu_sv_scale(n, dtype_code(v.dtype), _ptr(v), float(s), _stream(v.device)), "scale")
+
+
+def convert(n: int, src: torch.Tensor, dst: torch.Tensor) -> torch.Tensor:
+    _require_device_vec(src, "convert")
+    _require_device_vec(dst, "convert")
+    check(LIB.emu_sv_convert(n, dtype_code(src.dtype), _ptr(src), dtype_code(dst.dtype), _ptr(dst),
+                             _stream(src.device)), "convert")
+    return dst
+
+
+def correlation(n: int, psi: torch.Tensor) -> torch.Tensor:
+    """n x n float64 device tensor <n_i n_j> (qubit order)."""
+    _require_device_vec(psi, "correlation")
+    ws = Workspace.get(psi.device, n)
+    out = torch.empty(n, n, dtype=torch.float64, device=psi.device)
+    check(LIB.emu_sv_correlation(n, dtype_code(psi.dtype), _ptr(psi), _ptr(out), _ptr(ws.scratch),
+                                 _stream(psi.device)), "correlation")
+    return out
+
+
+def expm(a: np.ndarray) -> np.ndarray:
+    """Host complex128 matrix exponential (Pade 13, scaling and squaring)."""
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+    m = a.shape[0]
+    if a.shape != (m, m):
+        raise ValueError("expm: square matrix expected")
+    e = np.empty_like(a)
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(LIB.emu_sv_expm(m, a.view(np.float64).ctypes.data_as(dp),
+                          e.view(np.float64).ctypes.data_as(dp)), "expm")
+    return e

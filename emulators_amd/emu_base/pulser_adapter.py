@@ -1,0 +1,217 @@
+"""Inputs of the state-vector path: SequenceData and its pulse sampling.
+
+The reference builds ``SequenceData`` from a pulser ``Sequence``
+(emu_base/pulser_adapter.py:283-405).  pulser is not a dependency here; this
+module provides the same ``SequenceData`` contract plus a pulser-free
+producer from per-qubit 1 ns samples (the representation pulser's sampler
+returns), with the reference's semantics:
+
+* target times: a uniform dt grid plus the observables' evaluation times,
+  near-duplicates collapsed within TIME_TOLERANCE (pulser_adapter.py:80-152);
+* Omega/delta/phi at step midpoints by PCHIP over t = 0..T-1, the last
+  amplitude value clamped at >= 0, all cast to complex128
+  (pulser_adapter.py:155-231);
+* the interaction matrix as a callable of time, masked before the SLM end
+  time (pulser_adapter.py:234-256).
+"""
+
+from __future__ import annotations
+
+import bisect
+import math
+from dataclasses import dataclass
+from enum import Enum
+from typing import Callable, Iterable, Mapping, Sequence
+
+import torch
+
+from emulators_amd.emu_base.math.pchip_torch import PCHIP1D
+
+# Two times closer than this are the same time.  The reference takes pulser's
+# constant; its tests bound it to [1e-13, 1e-9) (test/emu_base/test_adapter.py
+# near-duplicate and evaluation-time tests).
+TIME_TOLERANCE = 1e-10
+
+
+class HamiltonianType(Enum):
+    Rydberg = 1
+    XY = 2
+
+
+@dataclass(frozen=True)
+class SequenceData:
+    """Per-run inputs of the backend (fields as pulser_adapter.py:259-280)."""
+
+    omega: torch.Tensor
+    delta: torch.Tensor
+    phi: torch.Tensor
+    interaction_matrix: Callable[[float], torch.Tensor]
+    qubit_ids: tuple
+    bad_atoms: tuple
+    lindblad_ops: list
+    state_prep_error: float
+    target_times: list
+    eigenstates: list
+    hamiltonian_type: HamiltonianType
+    has_stochastic_noise: bool
+
+    @property
+    def qubit_count(self) -> int:
+        return len(self.qubit_ids)
+
+    @property
+    def dim(self) -> int:
+        return len(self.eigenstates)
+
+
+def _fuzzy_union(base: Sequence[float], extra: Iterable[float], tol: float) -> list[float]:
+    """Insert the sorted ``extra`` times into the sorted ``base`` unless a time
+    already present lies within ``tol`` (base times win)."""
+    merged = list(base)
+    for t in extra:
+        i = bisect.bisect_left(merged, t)
+        near = (i > 0 and t - merged[i - 1] <= tol) or (i < len(merged) and merged[i] - t <= tol)
+        if not near:
+            merged.insert(i, t)
+    return merged
+
+
+def unique_observable_times(config) -> set[float]:
+    """All relative evaluation times of the config's observables."""
+    times: set[float] = set()
+    for obs in config.observables:
+        if obs.evaluation_times is not None:
+            times |= {float(t) for t in obs.evaluation_times}
+        elif not isinstance(config.default_evaluation_times, str):
+            times |= {float(t) for t in config.default_evaluation_times}
+        else:
+            raise ValueError(
+                f"default config {config.default_evaluation_times} is not supported."
+            )
+    return times
+
+
+def get_target_times(duration: float, config, dt: float) -> list[float]:
+    """Absolute sampling times of a run of ``duration`` ns (reference
+    _get_target_times, with the duration given directly)."""
+    duration = float(duration)
+    n_steps = math.floor(duration / dt)
+    rel = [i * float(dt) / duration for i in range(n_steps + 1)]
+    if rel[-1] < 1.0 - TIME_TOLERANCE:
+        rel.append(1.0)
+    rel = _fuzzy_union(rel, sorted(unique_observable_times(config)), TIME_TOLERANCE)
+    return [t * duration for t in rel]
+
+
+def is_evaluation_time(config, observable, t: float, tolerance: float = TIME_TOLERANCE) -> bool:
+    """True if relative time ``t`` is one of the observable's own evaluation
+    times or, when it has none, of the config's default times."""
+    times = observable.evaluation_times
+    if times is None:
+        times = config.default_evaluation_times
+    if isinstance(times, str):
+        return True
+    return any(abs(float(t) - float(x)) <= tolerance for x in times)
+
+
+def extract_omega_delta_phi(samples: Mapping[object, Mapping[str, Sequence[float]]],
+                            qubit_ids: Sequence, target_times: Sequence[float]
+                            ) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Midpoint (Omega, delta, phi) tables, shape (steps, qubits), complex128.
+
+    ``samples[qid]`` holds 1 ns samples "amp", "det", "phase" on t = 0..T-1.
+    Qubits absent from ``samples`` get zeros (pulser_adapter.py:189-199).
+    """
+    tt = torch.as_tensor(target_times, dtype=torch.float64)
+    t_mid = 0.5 * (tt[:-1] + tt[1:])
+    shape = (t_mid.numel(), len(qubit_ids))
+    out = {name: torch.zeros(shape, dtype=torch.float64) for name in ("amp", "det", "phase")}
+    t_grid = torch.arange(target_times[-1], dtype=torch.float64)
+    for q_pos, q_id in enumerate(qubit_ids):
+        if q_id not in samples:
+            continue
+        for name, table in out.items():
+            signal = torch.as_tensor(samples[q_id][name])
+            if torch.is_complex(signal):
+                if not torch.allclose(signal.imag, torch.zeros_like(signal.imag)):
+                    raise ValueError(f"Input {name} has non-zero imaginary part.")
+                signal = signal.real
+            signal = signal.to(torch.float64)
+            if signal.numel() != t_grid.numel():
+                raise ValueError(
+                    f"{name} of {q_id}: {signal.numel()} samples for a {t_grid.numel()} ns run"
+                )
+            table[:, q_pos] = PCHIP1D(t_grid, signal)(t_mid)
+            if name == "amp":
+                table[-1, q_pos] = torch.clamp(table[-1, q_pos], min=0.0)
+    return tuple(out[k].to(torch.complex128) for k in ("amp", "det", "phase"))  # type: ignore
+
+
+class InteractionMatrixCallable:
+    """U(t): the SLM-masked matrix before ``slm_end_time``, the full one after."""
+
+    def __init__(self, full_matrix: torch.Tensor, masked_matrix: torch.Tensor,
+                 slm_end_time: float):
+        self.full_matrix = full_matrix
+        self.masked_matrix = masked_matrix
+        self.slm_end_time = slm_end_time
+
+    def __call__(self, t: float) -> torch.Tensor:
+        return self.masked_matrix if t < self.slm_end_time else self.full_matrix
+
+
+def interaction_matrix_from_positions(positions, c6: float) -> torch.Tensor:
+    """U_ij = C6 / r_ij^6 (rad/us, positions in um), zero diagonal, (N, N)."""
+    pos = torch.as_tensor(positions, dtype=torch.float64)
+    r = torch.cdist(pos, pos)
+    u = torch.where(r > 0, c6 / r.clamp_min(1e-300) ** 6, torch.zeros_like(r))
+    u.fill_diagonal_(0.0)
+    return u
+
+
+def sequence_data_from_samples(samples: Mapping[object, Mapping[str, Sequence[float]]],
+                               qubit_ids: Sequence, interaction: torch.Tensor, config,
+                               dt: float, slm_targets: Sequence[int] = (),
+                               slm_end_time: float = 0.0) -> SequenceData:
+    """SequenceData from 1 ns samples and an (N, N) interaction matrix.
+
+    Applies ``config.interaction_cutoff`` and the SLM mask as
+    PulserData.get_sequences does (pulser_adapter.py:358-405)."""
+    lengths = {len(s[name]) for s in samples.values() for name in ("amp", "det", "phase")}
+    if len(lengths) != 1:
+        raise ValueError("all sample arrays must share one duration")
+    duration = lengths.pop()
+    target_times = get_target_times(duration, config, dt)
+    omega, delta, phi = extract_omega_delta_phi(samples, qubit_ids, target_times)
+    full = torch.as_tensor(interaction, dtype=torch.float64).clone().reshape(1, len(qubit_ids), -1)
+    full[torch.abs(full) < config.interaction_cutoff] = 0.0
+    masked = full.clone()
+    for tgt in slm_targets:
+        masked[:, tgt] = 0.0
+        masked[:, :, tgt] = 0.0
+    return SequenceData(
+        omega=omega, delta=delta, phi=phi,
+        interaction_matrix=InteractionMatrixCallable(full, masked, slm_end_time),
+        qubit_ids=tuple(qubit_ids), bad_atoms=tuple(False for _ in qubit_ids),
+        lindblad_ops=[], state_prep_error=0.0, target_times=target_times,
+        eigenstates=["r", "g"], hamiltonian_type=HamiltonianType.Rydberg,
+        has_stochastic_noise=False,
+    )
+
+
+class PulserData:
+    """SequenceData producer from a pulser Sequence (needs pulser)."""
+
+    def __init__(self, *, sequence, config, dt: float):
+        try:
+            import pulser  # noqa: F401
+        except ImportError as exc:  # pragma: no cover - pulser absent in this image
+            raise ImportError(
+                "SVBackend.run() on a pulser Sequence needs pulser; use "
+                "SVBackend._run_from_sequence_data(SequenceData, SVConfig) or "
+                "emulators_amd.emu_base.sequence_data_from_samples instead"
+            ) from exc
+        raise NotImplementedError(
+            "pulser Sequence sampling is not part of this build; build SequenceData with "
+            "sequence_data_from_samples"
+        )

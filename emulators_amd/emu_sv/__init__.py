@@ -1,0 +1,41 @@
+"""State-vector backend: drop-in for the reference's emu_sv on MI355X."""
+
+from .dense_state import DenseState
+from .hamiltonian import RydbergHamiltonian
+from .observables import (
+    BitStrings,
+    CorrelationMatrix,
+    Energy,
+    EnergySecondMoment,
+    EnergyVariance,
+    Fidelity,
+    Observable,
+    Occupation,
+    Results,
+    StateResult,
+)
+from .state_vector import StateVector, inner
+from .sv_backend import SVBackend
+from .sv_config import Solver, SVConfig
+
+__all__ = [
+    "BitStrings",
+    "CorrelationMatrix",
+    "DenseState",
+    "Energy",
+    "EnergySecondMoment",
+    "EnergyVariance",
+    "Fidelity",
+    "Observable",
+    "Occupation",
+    "Results",
+    "RydbergHamiltonian",
+    "SVBackend",
+    "SVConfig",
+    "Solver",
+    "StateResult",
+    "StateVector",
+    "inner",
+]
+
+__version__ = "0.1.0"

@@ -1,0 +1,94 @@
+"""Dense states (reference emu_sv/dense_state.py:17-240, state-vector part)."""
+
+from __future__ import annotations
+
+import math
+from collections import Counter
+from typing import Mapping, Sequence, Type, TypeVar
+
+import torch
+
+from emulators_amd.emu_base import DEVICE_COUNT, apply_measurement_errors
+from emulators_amd.emu_sv.utils import index_to_bitstring, sample_indices
+
+dtype = torch.complex128
+DenseStateType = TypeVar("DenseStateType", bound="DenseState")
+
+
+class DenseState:
+    """A complex128 vector over the qubit basis; index bit N-1-q is qubit q,
+    bit value 1 = "r" (dense_state.py:183-185)."""
+
+    def __init__(self, data: torch.Tensor, *, gpu: bool = True,
+                 eigenstates: Sequence[str] = ("r", "g")):
+        assert data.dim() == 1 or data.dim() == 2
+        self.eigenstates = tuple(eigenstates)
+        device = "cuda" if gpu and DEVICE_COUNT > 0 else "cpu"
+        self.data = data.to(dtype=dtype, device=device)
+
+    def __repr__(self) -> str:
+        return repr(self.data)
+
+    @property
+    def n_qudits(self) -> int:
+        return int(math.log2(self.data.shape[0]))
+
+    def sample(self, num_shots: int = 1000, one_state=None, p_false_pos: float = 0.0,
+               p_false_neg: float = 0.0) -> Counter:
+        """Bitstring counts drawn from |psi|^2 (dense_state.py:40-95)."""
+        if self.data.dim() == 2:
+            probabilities = torch.abs(self.data.diagonal())
+        else:
+            probabilities = torch.abs(self.data) ** 2
+        outcomes = sample_indices(probabilities, num_shots)
+        counts = Counter(index_to_bitstring(self.n_qudits, int(o)) for o in outcomes.tolist())
+        if p_false_neg > 0 or p_false_pos > 0:
+            counts = apply_measurement_errors(counts, p_false_pos=p_false_pos,
+                                              p_false_neg=p_false_neg)
+        return counts
+
+    def overlap(self, other: "DenseState") -> torch.Tensor:
+        """|<a|b>|^2 for two vectors, Tr(a^dag b) with density matrices
+        (dense_state.py:97-140)."""
+        assert isinstance(other, DenseState), "Only overlaps between emu-sv states are implemented."
+        l_data, r_data = self.data, other.data
+        if l_data.dim() < r_data.dim():
+            l_data = torch.outer(l_data, l_data.conj())
+        if r_data.dim() < l_data.dim():
+            r_data = torch.outer(r_data, r_data.conj())
+        assert l_data.shape == r_data.shape, "States do not have the same shape."
+        ov = torch.vdot(l_data.flatten(), r_data.to(self.data.device).flatten())
+        if l_data.dim() == 1:
+            ov = torch.abs(ov) ** 2
+        return ov
+
+    @staticmethod
+    def _build_state_vector(n_qudits: int, amplitudes: Mapping[str, complex]) -> torch.Tensor:
+        acc = torch.zeros(2**n_qudits, dtype=torch.complex128,
+                          device="cuda" if torch.cuda.is_available() else "cpu")
+        for bits, amp in amplitudes.items():
+            acc[int(bits.replace("r", "1").replace("g", "0"), 2)] = amp
+        acc /= acc.norm()
+        return acc
+
+    @classmethod
+    def _from_state_amplitudes(cls: Type[DenseStateType], *, eigenstates: Sequence[str],
+                               n_qudits: int, amplitudes: Mapping[str, complex]):
+        basis = set(eigenstates)
+        if basis == {"0", "1"}:
+            raise NotImplementedError(
+                "{'0','1'} basis is related to XY Hamiltonian, which is not implemented"
+            )
+        if basis != {"r", "g"}:
+            raise ValueError("Unsupported basis provided")
+        sv = cls._build_state_vector(n_qudits=n_qudits, amplitudes=amplitudes)
+        return cls(sv, gpu=sv.is_cuda, eigenstates=eigenstates), amplitudes
+
+    @classmethod
+    def from_state_amplitudes(cls: Type[DenseStateType], *, eigenstates: Sequence[str],
+                              amplitudes: Mapping[str, complex]) -> DenseStateType:
+        """State from a {bitstring: amplitude} mapping, normalised."""
+        n = len(next(iter(amplitudes)))
+        state, _ = cls._from_state_amplitudes(eigenstates=eigenstates, n_qudits=n,
+                                              amplitudes=amplitudes)
+        return state

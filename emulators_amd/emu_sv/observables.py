@@ -1,0 +1,235 @@
+"""Observables and Results.
+
+The reference takes these from pulser.backend (``Observable``, ``Results``,
+``Occupation``, ``Energy`` ...), which is not a dependency here.  This module
+provides the subset the state-vector backend uses, with pulser's names,
+tags and call protocol:
+
+* ``obs(config, t, state, hamiltonian, results)`` evaluates ``obs.apply``
+  when the relative time ``t`` is one of the observable's evaluation times
+  (or of ``config.default_evaluation_times`` when it has none) and stores
+  the value in ``results`` under ``obs.tag``;
+* ``results.<tag>`` is the list of stored values in time order,
+  ``results.get_result_times(tag)`` their relative times.
+"""
+
+from __future__ import annotations
+
+import copy
+import uuid
+from collections import Counter
+from typing import Any, Sequence
+
+import numpy as np
+import torch
+
+from emulators_amd.emu_base.pulser_adapter import is_evaluation_time
+from emulators_amd.emu_sv import custom_callback_implementations as impl
+
+
+class AggregationMethod:
+    MEAN = "mean"
+    SKIP = "skip"
+    BAG_UNION = "bag_union"
+
+
+class Observable:
+    def __init__(self, *, evaluation_times: Sequence[float] | None = None,
+                 tag_suffix: str | None = None,
+                 default_aggregation_method: str = AggregationMethod.MEAN):
+        self.evaluation_times = (None if evaluation_times is None
+                                 else np.asarray(evaluation_times, dtype=np.float64))
+        self.tag_suffix = tag_suffix
+        self.default_aggregation_method = default_aggregation_method
+        self._uuid = uuid.uuid4()
+
+    @property
+    def _base_tag(self) -> str:
+        raise NotImplementedError
+
+    @property
+    def tag(self) -> str:
+        return self._base_tag if not self.tag_suffix else f"{self._base_tag}_{self.tag_suffix}"
+
+    @property
+    def uuid(self):
+        return self._uuid
+
+    def apply(self, *, config, state, hamiltonian) -> Any:
+        raise NotImplementedError
+
+    def __call__(self, config, t: float, state, hamiltonian, result: "Results") -> None:
+        if not is_evaluation_time(config, self, t):
+            return
+        value = self.apply(config=config, state=state, hamiltonian=hamiltonian)
+        result._store(observable=self, time=t, value=value)
+
+
+class Occupation(Observable):
+    _base_tag = "occupation"
+
+    def apply(self, *, config, state, hamiltonian):
+        return impl.qubit_occupation_sv_impl(self, config=config, state=state,
+                                             hamiltonian=hamiltonian)
+
+
+class CorrelationMatrix(Observable):
+    _base_tag = "correlation_matrix"
+
+    def apply(self, *, config, state, hamiltonian):
+        return impl.correlation_matrix_sv_impl(self, config=config, state=state,
+                                               hamiltonian=hamiltonian)
+
+
+class Energy(Observable):
+    _base_tag = "energy"
+
+    def apply(self, *, config, state, hamiltonian):
+        return hamiltonian.expect(state)
+
+
+class EnergyVariance(Observable):
+    _base_tag = "energy_variance"
+
+    def apply(self, *, config, state, hamiltonian):
+        return impl.energy_variance_sv_impl(self, config=config, state=state,
+                                            hamiltonian=hamiltonian)
+
+
+class EnergySecondMoment(Observable):
+    _base_tag = "energy_second_moment"
+
+    def apply(self, *, config, state, hamiltonian):
+        return impl.energy_second_moment_sv_impl(self, config=config, state=state,
+                                                 hamiltonian=hamiltonian)
+
+
+class Fidelity(Observable):
+    """|<state|psi(t)>|^2 against a fixed state."""
+
+    _base_tag = "fidelity"
+
+    def __init__(self, state, **kwargs):
+        super().__init__(**kwargs)
+        self.state = state
+
+    def apply(self, *, config, state, hamiltonian):
+        return self.state.overlap(state)
+
+
+class StateResult(Observable):
+    _base_tag = "state"
+
+    def __init__(self, **kwargs):
+        kwargs.setdefault("default_aggregation_method", AggregationMethod.SKIP)
+        super().__init__(**kwargs)
+
+    def apply(self, *, config, state, hamiltonian):
+        return copy.deepcopy(state)
+
+
+class BitStrings(Observable):
+    _base_tag = "bitstrings"
+
+    def __init__(self, *, num_shots: int = 1000, one_state=None, **kwargs):
+        kwargs.setdefault("default_aggregation_method", AggregationMethod.BAG_UNION)
+        super().__init__(**kwargs)
+        self.num_shots = num_shots
+        self.one_state = one_state
+
+    def apply(self, *, config, state, hamiltonian) -> Counter:
+        p_pos = getattr(config, "p_false_pos", 0.0)
+        p_neg = getattr(config, "p_false_neg", 0.0)
+        return state.sample(num_shots=self.num_shots, one_state=self.one_state,
+                            p_false_pos=p_pos, p_false_neg=p_neg)
+
+
+class Results:
+    """Values of the observables of one run, keyed by tag and time."""
+
+    def __init__(self, atom_order: Sequence, total_duration: int):
+        self.atom_order = tuple(atom_order)
+        self.total_duration = total_duration
+        self._results: dict = {}
+        self._tagmap: dict = {}
+        self._aggregation: dict = {}
+
+    def _store(self, *, observable: Observable, time: float, value: Any) -> None:
+        tag = observable.tag
+        if tag in self._tagmap and self._tagmap[tag] != observable.uuid:
+            raise ValueError(f"two observables share the tag {tag!r}")
+        self._tagmap[tag] = observable.uuid
+        self._aggregation[tag] = observable.default_aggregation_method
+        self._results.setdefault(tag, {})[float(time)] = value
+
+    def get_result_times(self, tag: str) -> list[float]:
+        return sorted(self._results[tag].keys())
+
+    def get_result(self, tag: str, time: float) -> Any:
+        times = self._results[tag]
+        best = min(times, key=lambda t: abs(t - time))
+        if abs(best - time) > 1e-10:
+            raise ValueError(f"no result of {tag!r} at time {time}")
+        return times[best]
+
+    def get_tagged_results(self) -> dict:
+        return {tag: [self._results[tag][t] for t in self.get_result_times(tag)]
+                for tag in self._results}
+
+    def __getattr__(self, name: str):
+        results = self.__dict__.get("_results", {})
+        if name in results:
+            return [results[name][t] for t in sorted(results[name])]
+        raise AttributeError(f"{type(self).__name__!r} has no attribute or tag {name!r}")
+
+    @staticmethod
+    def aggregate(results: Sequence["Results"]) -> "Results":
+        """One run: itself.  Several: the mean of each numeric observable,
+        the union of bitstring bags; SKIP observables are dropped."""
+        results = list(results)
+        if len(results) == 1:
+            return results[0]
+        first = results[0]
+        out = Results(first.atom_order, first.total_duration)
+        for tag, method in first._aggregation.items():
+            if method == AggregationMethod.SKIP:
+                continue
+            for t in first.get_result_times(tag):
+                vals = [r._results[tag][t] for r in results]
+                if method == AggregationMethod.BAG_UNION:
+                    val = sum(vals, Counter())
+                else:
+                    val = sum(vals) / len(vals)
+                out._results.setdefault(tag, {})[t] = val
+            out._tagmap[tag] = first._tagmap[tag]
+            out._aggregation[tag] = method
+        return out
+
+
+class Statistics(Observable):
+    """Per-step wall time and peak memory (sv_backend_impl.py:32-71)."""
+
+    _base_tag = "statistics"
+
+    def __init__(self, evaluation_times, data: list, timestep_count: int):
+        super().__init__(evaluation_times=evaluation_times,
+                         default_aggregation_method=AggregationMethod.SKIP)
+        self.data = data
+        self.timestep_count = timestep_count
+
+    def apply(self, *, config, state, hamiltonian) -> dict:
+        import logging
+
+        from emulators_amd.emu_base import get_max_rss_cpu, get_max_rss_gpu
+
+        duration = self.data[-1]
+        max_mem = get_max_rss_gpu() if state.data.is_cuda else get_max_rss_cpu()
+        logging.getLogger("emulators").info(
+            f"step = {len(self.data)}/{self.timestep_count}, RSS = {max_mem:.3f} MB, "
+            f"Δt = {duration:.3f} s"
+        )
+        return {"RSS": max_mem, "duration": duration}
+
+
+def as_float(x: Any) -> float:
+    return float(x.item() if isinstance(x, torch.Tensor) else x)

@@ -1,0 +1,145 @@
+/*
+ * emu_sv_hip.h -- C ABI of the MI355X (gfx950) state-vector Krylov engine.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and never
+ * allocates or frees a caller buffer.  Device pointers are owned by the
+ * caller (e.g. torch tensors).  Calls that take a hipStream_t only enqueue
+ * work on that stream; the caller synchronises.  Return value: 0 on success,
+ * a negative EMU_SV_E* code otherwise (emu_sv_last_error() gives the text).
+ *
+ * Conventions shared with the reference emulator (paths relative to the
+ * reference checkout):
+ *   - state: 1-D contiguous complex vector of length D = 2^N, interleaved
+ *     (re, im); complex128 ("c128") or complex64 ("c64").
+ *   - qubit q <-> bit (N-1-q) of the amplitude index: qubit 0 is the most
+ *     significant bit (emu_sv/hamiltonian.py:107); bit value 1 = Rydberg "r".
+ *   - H = sum_q (O_q/2)[cos(phi_q) X_q + sin(phi_q) Y_q] - sum_q d_q n_q
+ *         + sum_{i<j} U_ij n_i n_j   (emu_sv/hamiltonian.py:11); only the
+ *     upper triangle U[i][j], i<j, of the interaction matrix is read
+ *     (emu_sv/hamiltonian.py:157-161).
+ *   - the Krylov operator is op(x) = sigma * H x with sigma = -i dt
+ *     (emu_sv/time_evolution/torch.py:161-162).
+ */
+#ifndef EMU_SV_HIP_H
+#define EMU_SV_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+#define EMU_SV_OK 0
+#define EMU_SV_EINVAL -1        /* bad size / shape / argument -> ValueError   */
+#define EMU_SV_EHIP -2          /* HIP runtime error           -> RuntimeError */
+#define EMU_SV_ENOTCONVERGED -3 /* Krylov not converged        -> RecursionError */
+
+/* element types */
+#define EMU_SV_C128 0
+#define EMU_SV_C64 1
+
+/* emu_sv_hpsi flags */
+#define EMU_SV_FLAG_COMPLEX 1 /* some phi != 0: complex flip coefficients */
+
+#define EMU_SV_MAX_QUBITS 32
+/* Packed Hamiltonian parameter block, indexed by BIT position b = N-1-q:
+ *   [0,32)        Re c_b, c_b = (O_q/2) e^{i phi_q}  (<1|H|0> on that qubit)
+ *   [32,64)       Im c_b
+ *   [64,96)       d_b (detuning)
+ *   [96,96+1024)  symmetric U_b[b][b'] (upper triangle of the qubit matrix
+ *                 mirrored, zero diagonal)
+ *   [1120]        flag: 1.0 if any Im c_b != 0 (the reference's complex
+ *                 sigma path, emu_sv/hamiltonian.py:61)            */
+#define EMU_SV_HAM_DOUBLES 1128
+
+/* Last error message of the calling thread ("" if none). */
+const char *emu_sv_last_error(void);
+
+/* Library version string. */
+const char *emu_sv_version(void);
+
+/* Pack (O, d, phi, U) given per QUBIT into the parameter block above
+ * (host memory).  omega/delta/phi: n doubles; interaction: n*n row-major
+ * doubles (only i<j read).  Replaces RydbergHamiltonian.__init__ and the
+ * diagonal it stores (emu_sv/hamiltonian.py:42-67, 139-162): the diagonal is
+ * never materialised, kernels rebuild it on the fly from this block. */
+int emu_sv_pack_hamiltonian(int n, const double *omega, const double *delta,
+                            const double *phi, const double *interaction,
+                            double *block_out);
+
+/* Number of H.psi passes the planner uses for n qubits and element type
+ * (each pass streams the vector once through LDS tiles). */
+int emu_sv_plan_passes(int n, int dtype, int tile_log, int min_contig);
+
+/* Bytes of the per-workgroup partial-sum workspace needed by the calls
+ * below for a vector of 2^n elements. */
+size_t emu_sv_workspace_bytes(int n, int dtype);
+
+/* w = sigma*H*(v_scale*v)  [+ reductions].  Replaces
+ * RydbergHamiltonian.__mul__ (emu_sv/hamiltonian.py:69-137) composed with the
+ * -1j*dt scaling of op (time_evolution/torch.py:161-162) and fused with the
+ * dot products of one Lanczos iteration (emu_base/math/krylov_exp.py:52-58).
+ *   ham      device pointer to a packed parameter block
+ *   v, w     device vectors (must not alias); prev may be NULL
+ *   dots_out device pointer to 7 doubles receiving, after the call:
+ *            [0] ||w||^2, [1,2] <v|w>, [3,4] <prev|w>, [5,6] <v|prev>
+ *            (v and prev taken with their scales); NULL = no reductions.
+ *   workspace device scratch of emu_sv_workspace_bytes(n, dtype) bytes.
+ *   tile_log / min_contig: LDS tile size and minimum contiguous bits of
+ *            the high-bit passes; 0 = library default.
+ *   flags    EMU_SV_FLAG_COMPLEX when any phi != 0 (emu_sv/hamiltonian.py:61).
+ */
+int emu_sv_hpsi(int n, int dtype, const double *ham, double sigma_re,
+                double sigma_im, const void *v, double v_scale,
+                const void *prev, double prev_scale, void *w, double *dots_out,
+                void *workspace, int tile_log, int min_contig, int flags,
+                hipStream_t stream);
+
+/* w <- w - a*(pa_scale*pa) - b*(pb_scale*pb); norm2_out[0] = ||w||^2.
+ * The orthogonalisation of krylov_exp.py:54-61 (two-vector MGS, with the
+ * coefficients computed by emu_sv_hpsi's fused dots). pa/pb may be NULL. */
+int emu_sv_lanczos_update(int n, int dtype, void *w, const void *pa,
+                          double pa_scale, double a_re, double a_im,
+                          const void *pb, double pb_scale, double b_re,
+                          double b_im, double *norm2_out, void *workspace,
+                          hipStream_t stream);
+
+/* out = sum_k coef_k * V_k for k < count (count <= 1024).  coef: host array of
+ * 2*count doubles (re, im), already including vector scales.  V: host array
+ * of device pointers.  out may alias V[0] (and no other V_k).  The final combination of
+ * krylov_exp.py:66,88-90. */
+int emu_sv_lincomb(int n, int dtype, int count, const void *const *vecs,
+                   const double *coef, void *out, hipStream_t stream);
+
+/* dots_out[0..2] = ||a||^2, Re<a|b>, Im<a|b> (b may be NULL).  Replaces the
+ * vector_norm / vdot / tensordot calls of the reference on the path. */
+int emu_sv_dot(int n, int dtype, const void *a, const void *b, double *dots_out,
+               void *workspace, hipStream_t stream);
+
+/* v <- s * v (real s). */
+int emu_sv_scale(int n, int dtype, void *v, double s, hipStream_t stream);
+
+/* Copy with element-type conversion (c128 <-> c64 or same type). */
+int emu_sv_convert(int n, int src_dtype, const void *src, int dst_dtype,
+                   void *dst, hipStream_t stream);
+
+/* Density-density correlation <n_i n_j>, n x n float64 into corr_out
+ * (device), qubit-indexed; its diagonal is the occupation <n_i>.
+ * Replaces qubit_occupation_sv_impl and correlation_matrix_sv_impl
+ * (emu_sv/custom_callback_implementations.py:30-47, 77-102). */
+int emu_sv_correlation(int n, int dtype, const void *psi, double *corr_out,
+                       void *workspace, hipStream_t stream);
+
+/* Host: E = expm(A) for an m x m complex128 matrix (row-major, interleaved),
+ * scaling and squaring with a [13/13] Pade approximant (Higham 2005).
+ * Replaces torch.linalg.matrix_exp on the (j+3)x(j+3) extended Krylov
+ * matrix (emu_base/math/krylov_exp.py:65,78). */
+int emu_sv_expm(int m, const double *a, double *e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EMU_SV_HIP_H */

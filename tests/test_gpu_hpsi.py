@@ -1,0 +1,175 @@
+"""H.psi through the C ABI against the oracle (parity, bit-level tolerances).
+
+complex128: |HIP - oracle| <= 1e-12 * ||H psi|| (the reference compares its
+sparse and dense H.psi at atol 1e-12, test/emu_sv/test_hamiltonian.py:51).
+complex64: relative error <= 2e-6 (FP32 vectors, FP64 diagonal).
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle.emu_sv_oracle import OracleHamiltonian, dense_rydberg_hamiltonian
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(n, with_phase, seed):
+    g = torch.Generator().manual_seed(seed)
+    om = torch.randn(n, dtype=torch.float64, generator=g) * 3
+    de = torch.randn(n, dtype=torch.float64, generator=g) * 3
+    ph = torch.randn(n, dtype=torch.float64, generator=g) if with_phase else torch.zeros(n, dtype=torch.float64)
+    u = torch.rand(n, n, dtype=torch.float64, generator=g) * 5
+    u = torch.triu(u, 1)
+    u = u + u.T
+    return om, de, ph, u
+
+
+def _hpsi_gpu(n, om, de, ph, u, psi, dtype, sigma=1.0, tile_log=0, min_contig=0):
+    from emulators_amd import _native as native
+    block, cplx = native.pack_hamiltonian(om.numpy(), de.numpy(), ph.numpy(), u.numpy())
+    dev = torch.device("cuda")
+    params = torch.from_numpy(block).to(dev)
+    v = psi.to(dev, dtype)
+    w = torch.empty_like(v)
+    native.hpsi(n, params, sigma, v, 1.0, w, complex_coef=cplx, tile_log=tile_log,
+                min_contig=min_contig)
+    return w.to("cpu", torch.complex128)
+
+
+@pytest.mark.parametrize("with_phase", [False, True])
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 8, 9, 10, 11, 12, 13, 14])
+def test_hpsi_c128_vs_oracle(gpu_device, n, with_phase):
+    om, de, ph, u = _params(n, with_phase, 1000 + n)
+    psi = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(n))
+    ref = OracleHamiltonian(om, de, ph, u) * psi
+    got = _hpsi_gpu(n, om, de, ph, u, psi, torch.complex128)
+    assert torch.allclose(got, ref, atol=1e-12 * max(1.0, float(ref.norm())), rtol=0)
+
+
+@pytest.mark.parametrize("with_phase", [False, True])
+@pytest.mark.parametrize("n", [3, 8, 12, 15])
+def test_hpsi_c64_vs_oracle(gpu_device, n, with_phase):
+    om, de, ph, u = _params(n, with_phase, 2000 + n)
+    psi = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(n))
+    psi32 = psi.to(torch.complex64).to(torch.complex128)
+    ref = OracleHamiltonian(om, de, ph, u) * psi32
+    got = _hpsi_gpu(n, om, de, ph, u, psi, torch.complex64)
+    err = float((got - ref).norm() / ref.norm())
+    assert err < 2e-6, err
+
+
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+@pytest.mark.parametrize("n,tile_log,min_contig", [
+    (9, 8, 0), (10, 8, 2), (12, 8, 3), (13, 8, 1), (14, 9, 4), (16, 8, 2), (17, 12, 3),
+    (18, 13, 3), (20, 10, 5), (21, 12, 2)])
+def test_hpsi_multipass_schedules(gpu_device, n, tile_log, min_contig, dtype):
+    """Small tiles force 2-4 passes at small n, covering every planner shape."""
+    from emulators_amd import _native as native
+    passes = native.plan_passes(n, dtype, tile_log, min_contig)
+    assert passes >= 2
+    om, de, ph, u = _params(n, True, 3000 + n)
+    psi = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(7 + n))
+    if dtype == torch.complex64:
+        psi = psi.to(torch.complex64).to(torch.complex128)
+    ref = OracleHamiltonian(om, de, ph, u) * psi
+    got = _hpsi_gpu(n, om, de, ph, u, psi, dtype, tile_log=tile_log, min_contig=min_contig)
+    tol = 1e-12 if dtype == torch.complex128 else 2e-6
+    err = float((got - ref).norm() / ref.norm())
+    assert err < tol, (err, passes)
+
+
+def test_hpsi_dense_and_sigma(gpu_device):
+    """sigma*H against the dense Kronecker H (test_hamiltonian.py:20-85 style)."""
+    n = 6
+    om, de, ph, u = _params(n, True, 77)
+    h = dense_rydberg_hamiltonian(om, de, ph, u)
+    psi = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(3))
+    sigma = -0.37j + 0.11
+    got = _hpsi_gpu(n, om, de, ph, u, psi, torch.complex128, sigma=sigma)
+    ref = sigma * (h @ psi)
+    assert torch.allclose(got, ref, atol=1e-12, rtol=0)
+
+
+def test_fused_dots(gpu_device):
+    from emulators_amd import _native as native
+    n = 11
+    om, de, ph, u = _params(n, True, 5)
+    block, cplx = native.pack_hamiltonian(om.numpy(), de.numpy(), ph.numpy(), u.numpy())
+    dev = torch.device("cuda")
+    params = torch.from_numpy(block).to(dev)
+    g = torch.Generator().manual_seed(11)
+    v = torch.randn(2**n, dtype=torch.complex128, generator=g)
+    p = torch.randn(2**n, dtype=torch.complex128, generator=g)
+    sv, sp, sigma = 0.7, 1.3, -0.02j
+    w = torch.empty(2**n, dtype=torch.complex128, device=dev)
+    d = native.hpsi(n, params, sigma, v.to(dev), sv, w, prev=p.to(dev), prev_scale=sp,
+                    dots=True, complex_coef=cplx).cpu().numpy()
+    ham = OracleHamiltonian(om, de, ph, u)
+    wr = sigma * (ham * (sv * v))
+    vv, pp = sv * v, sp * p
+    assert math.isclose(d[0], float(torch.vdot(wr, wr).real), rel_tol=1e-12)
+    assert abs(complex(d[1], d[2]) - complex(torch.vdot(vv, wr))) < 1e-12 * float(wr.norm() * vv.norm())
+    assert abs(complex(d[3], d[4]) - complex(torch.vdot(pp, wr))) < 1e-12 * float(wr.norm() * pp.norm())
+    assert abs(complex(d[5], d[6]) - complex(torch.vdot(vv, pp))) < 1e-12 * float(vv.norm() * pp.norm())
+    assert torch.allclose(w.cpu(), wr, atol=1e-12 * float(wr.norm()), rtol=0)
+
+
+def test_diag_property(gpu_device):
+    from emulators_amd.emu_sv import RydbergHamiltonian
+    n = 9
+    om, de, ph, u = _params(n, False, 9)
+    h = RydbergHamiltonian(om, de, ph, u, device="cuda", noise=None)
+    ref = OracleHamiltonian(om, de, ph, u).diag
+    assert torch.allclose(h.diag.cpu(), ref, atol=1e-12)
+
+
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_streaming_kernels(gpu_device, dtype):
+    from emulators_amd import _native as native
+    n = 15
+    dev = torch.device("cuda")
+    g = torch.Generator().manual_seed(21)
+    a = torch.randn(2**n, dtype=torch.complex128, generator=g).to(dtype)
+    b = torch.randn(2**n, dtype=torch.complex128, generator=g).to(dtype)
+    c = torch.randn(2**n, dtype=torch.complex128, generator=g).to(dtype)
+    tol = 1e-12 if dtype == torch.complex128 else 1e-6
+    a64, b64, c64 = (x.to(torch.complex128) for x in (a, b, c))
+    # dot
+    d = native.dot(n, a.to(dev), b.to(dev)).cpu().numpy()
+    assert math.isclose(d[0], float(a64.norm() ** 2), rel_tol=tol)
+    assert abs(complex(d[1], d[2]) - complex(torch.vdot(a64, b64))) < tol * float(a64.norm() * b64.norm())
+    # lanczos update
+    w = a.to(dev).clone()
+    al, be = 0.3 - 0.2j, -1.1 + 0.4j
+    nrm = native.lanczos_update(n, w, b.to(dev), 0.5, al, c.to(dev), 2.0, be).item()
+    ref = a64 - al * 0.5 * b64 - be * 2.0 * c64
+    assert torch.allclose(w.cpu().to(torch.complex128), ref, atol=tol * 10, rtol=tol * 10)
+    assert math.isclose(nrm, float(w.cpu().to(torch.complex128).norm() ** 2), rel_tol=tol)
+    # lincomb over > one batch of vectors, output aliasing the first
+    vecs = [torch.randn(2**n, dtype=torch.complex128, generator=g).to(dtype).to(dev) for _ in range(53)]
+    coef = np.random.default_rng(0).standard_normal(53) + 1j * np.random.default_rng(1).standard_normal(53)
+    ref = sum(complex(c_) * v.cpu().to(torch.complex128) for c_, v in zip(coef, vecs))
+    out = native.lincomb(n, vecs, coef, vecs[0])
+    assert torch.allclose(out.cpu().to(torch.complex128), ref, atol=tol * 50, rtol=tol * 10)
+    # scale / convert
+    x = a.to(dev).clone()
+    native.scale(n, x, 0.25)
+    assert torch.allclose(x.cpu(), a * 0.25)
+    other = torch.complex64 if dtype == torch.complex128 else torch.complex128
+    y = torch.empty(2**n, dtype=other, device=dev)
+    native.convert(n, a.to(dev), y)
+    assert torch.allclose(y.cpu().to(torch.complex128), a64.to(other).to(torch.complex128))
+
+
+@pytest.mark.parametrize("n", [1, 3, 8, 9, 14, 15, 17])
+def test_correlation(gpu_device, n):
+    from emulators_amd import _native as native
+    from oracle.emu_sv_oracle import correlation_matrix
+    psi = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(n))
+    psi /= psi.norm()
+    got = native.correlation(n, psi.to("cuda")).cpu()
+    ref = correlation_matrix(psi)
+    assert torch.allclose(got, ref, atol=1e-13, rtol=0)

@@ -1,0 +1,251 @@
+"""Krylov steps and whole runs through the C ABI against the oracle.
+
+Bars: step accuracy vs dense expm at atol = tolerance (the reference's
+test_time_evolution.py:58-76); Lanczos iteration counts equal to the oracle's
+on the same operator; whole runs with final-state fidelity >= 1 - 1e-10 and
+observables within 1e-9 (complex128).
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import emu_sv_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _randn_params(n, with_phase, seed):
+    torch.manual_seed(seed)
+    om = torch.randn(n, dtype=torch.float64)
+    de = torch.randn(n, dtype=torch.float64)
+    ph = torch.randn(n, dtype=torch.float64) if with_phase else torch.zeros(n, dtype=torch.float64)
+    u = torch.zeros(n, n, dtype=torch.float64)
+    for i in range(n):
+        for j in range(i + 1, n):
+            u[i, j] = u[j, i] = torch.rand(1, dtype=torch.float64)
+    return om, de, ph, u
+
+
+@pytest.mark.usefixtures("restore_rng_state")
+@pytest.mark.parametrize("n,tol,with_phase", [(n, t, p) for n in (3, 5, 8)
+                                              for t in (1e-8, 1e-10, 1e-12) for p in (False, True)])
+def test_forward_accuracy(gpu_device, n, tol, with_phase):
+    from emulators_amd.emu_sv.time_evolution import EvolveStateVector
+    om, de, ph, u = _randn_params(n, with_phase, 1337)
+    psi = torch.randn(2**n, dtype=torch.complex128)
+    psi = psi / psi.norm()
+    expected = orc.dense_step(1.0, om, de, ph, u, psi)
+    out, _ = EvolveStateVector().apply(1.0, om, de, ph, u, psi.to(gpu_device), tol, [])
+    assert torch.allclose(out.cpu(), expected, atol=tol, rtol=0)
+
+
+@pytest.mark.parametrize("n,with_phase", [(6, False), (10, True), (12, False), (14, True)])
+def test_iteration_count_matches_oracle(gpu_device, n, with_phase):
+    from emulators_amd.emu_base.math.krylov_exp import ScaledHamiltonian, krylov_exp_impl
+    from emulators_amd.emu_sv import RydbergHamiltonian
+    om, de, ph, u = _randn_params(n, with_phase, 99 + n)
+    om, de, u = om * 10, de * 5, u * 20
+    psi = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(n))
+    dt, tol = 0.05, 1e-10
+    ham_o = orc.OracleHamiltonian(om, de, ph, u)
+    ref = orc.krylov_exp_impl(lambda x: -1j * dt * (ham_o * x), psi.clone(), True, tol, tol)
+    ham = RydbergHamiltonian(om, de, ph, u, device=gpu_device)
+    got = krylov_exp_impl(ScaledHamiltonian(ham, -1j * dt), psi.to(gpu_device), True, tol, tol)
+    assert got.converged and ref.converged
+    assert got.iteration_count == ref.iteration_count
+    assert torch.allclose(got.result.cpu(), ref.result, atol=1e-12, rtol=0)
+
+
+def test_happy_breakdown_and_nonconvergence(gpu_device):
+    from emulators_amd.emu_base.math.krylov_exp import ScaledHamiltonian, krylov_exp, krylov_exp_impl
+    from emulators_amd.emu_sv import RydbergHamiltonian
+    n = 4
+    # Omega = 0: H is diagonal, |psi> = |0..0> is an eigenvector -> breakdown at 1
+    z = torch.zeros(n, dtype=torch.float64)
+    ham = RydbergHamiltonian(z, torch.ones(n, dtype=torch.float64), z,
+                             torch.zeros(n, n, dtype=torch.float64), device=gpu_device)
+    psi = torch.zeros(2**n, dtype=torch.complex128, device=gpu_device)
+    psi[0] = 2.0
+    r = krylov_exp_impl(ScaledHamiltonian(ham, -0.1j), psi, True, 1e-10, 1e-10)
+    assert r.happy_breakdown and r.iteration_count == 1
+    assert torch.allclose(r.result.cpu(), psi.cpu())
+    # too small a Krylov space -> RecursionError, as krylov_exp.py:126-129
+    om, de, ph, u = _randn_params(8, False, 4)
+    ham = RydbergHamiltonian(om * 30, de, ph, u, device=gpu_device)
+    v = torch.randn(2**8, dtype=torch.complex128, device=gpu_device)
+    with pytest.raises(RecursionError):
+        krylov_exp(ScaledHamiltonian(ham, -1j), v, 1e-10, 1e-10, max_krylov_dim=3)
+
+
+def test_complex64_step(gpu_device):
+    from emulators_amd.emu_sv.time_evolution import EvolveStateVector
+    n = 12
+    om, de, ph, u = _randn_params(n, False, 5)
+    om, u = om * 10, u * 30
+    psi = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(1))
+    psi /= psi.norm()
+    ref = orc.krylov_exp(lambda x: -1j * 0.01 * (orc.OracleHamiltonian(om, de, ph, u) * x),
+                         psi.clone(), 1e-10, 1e-10)
+    out, _ = EvolveStateVector().apply(0.01, om, de, ph, u, psi.to(gpu_device, torch.complex64),
+                                       1e-10, [])
+    assert out.dtype == torch.complex64
+    err = float((out.cpu().to(torch.complex128) - ref).norm())
+    assert err < 5e-6, err
+
+
+def _run_backend(seq, observables, precision="complex128"):
+    from emulators_amd.emu_sv import SVBackend, SVConfig
+    cfg = SVConfig(observables=observables, log_level=40, precision=precision)
+    return SVBackend._run_from_sequence_data(seq, cfg)
+
+
+def test_chain10_end_to_end_vs_oracle(gpu_device):
+    """The N=10 chain workload: final state fidelity >= 1 - 1e-10, observables
+    (occupation, energy) within 1e-9 of the complex128 oracle."""
+    from emulators_amd import workloads
+    from emulators_amd.emu_sv import Energy, Occupation, StateResult
+    seq = workloads.chain10()
+    res = _run_backend(seq, [StateResult(evaluation_times=[1.0]), Occupation(evaluation_times=[1.0]),
+                             Energy(evaluation_times=[1.0])])
+    ref = orc.run_sequence(seq.omega, seq.delta, seq.phi, seq.interaction_matrix, seq.target_times)
+    k = seq.omega.shape[0]
+    psi = res.state[-1].data.cpu()
+    fid = float(orc.fidelity(psi, ref.states[k]))
+    assert fid >= 1 - 1e-10, 1 - fid
+    assert torch.allclose(res.occupation[-1], ref.occupation[k], atol=1e-9, rtol=0)
+    assert abs(float(res.energy[-1]) - ref.energy[k]) < 1e-9 * max(1.0, abs(ref.energy[k]))
+
+
+def test_square20_complex64_fidelity(gpu_device):
+    """N=20 square lattice: complex64 vectors with FP64 Lanczos scalars against
+    the complex128 GPU run (itself checked against the oracle at small N)."""
+    from emulators_amd import workloads
+    from emulators_amd.emu_sv import Occupation, StateResult
+    seq = workloads.square20()
+    obs = lambda: [StateResult(evaluation_times=[1.0]), Occupation(evaluation_times=[1.0])]  # noqa: E731
+    r128 = _run_backend(seq, obs())
+    r64 = _run_backend(seq, obs(), precision="complex64")
+    a, b = r128.state[-1].data, r64.state[-1].data
+    fid = float(torch.abs(torch.vdot(a, b)) ** 2)
+    assert 1 - fid < 1e-6, 1 - fid
+    assert torch.max(torch.abs(r128.occupation[-1] - r64.occupation[-1])) < 1e-4
+
+
+def test_run_from_sequence_data_pi_pulse(gpu_device):
+    """The reference's SequenceData pi-pulse table (test_sv_backend.py:102-152):
+    occupation 0 at t=0 and 1 at the end."""
+    from emulators_amd.emu_base import HamiltonianType, SequenceData
+    from emulators_amd.emu_sv import Occupation, SVBackend, SVConfig
+    duration, dt = 100, 10
+    occup = Occupation(evaluation_times=[dt * x / duration for x in range(duration // dt + 1)])
+    cfg = SVConfig(observables=[occup], log_level=40, interaction_cutoff=1e-10)
+    col = [0.7092, 7.8431, 26.2913, 53.0011, 73.0656, 71.8630, 50.3238, 23.9187, 6.6745, 0.4483]
+    omega = torch.tensor([[c + 0j] * 3 for c in col], dtype=torch.complex128)
+    seq = SequenceData(
+        omega=omega, delta=torch.zeros_like(omega), phi=torch.zeros_like(omega),
+        interaction_matrix=lambda t: torch.zeros((1, 3, 3), dtype=torch.float64),
+        qubit_ids=("q0", "q1", "a2"), bad_atoms=(False, False, False), lindblad_ops=[],
+        state_prep_error=0.0, target_times=[dt * x for x in range(duration // dt + 1)],
+        eigenstates=["r", "g"], hamiltonian_type=HamiltonianType.Rydberg,
+        has_stochastic_noise=False)
+    results = SVBackend._run_from_sequence_data(seq, cfg)
+    assert set(results._tagmap.keys()) == {"occupation", "statistics"}
+    assert results.get_result_times("occupation") == occup.evaluation_times.tolist()
+    assert results.occupation[0] == pytest.approx(0.0)
+    assert results.occupation[-1] == pytest.approx(1.0)
+
+
+def test_observables_vs_oracle(gpu_device):
+    """Energy, variance, second moment and correlation on a random state."""
+    from emulators_amd.emu_sv import (CorrelationMatrix, EnergySecondMoment, EnergyVariance,
+                                      RydbergHamiltonian, StateVector)
+    n = 9
+    om, de, ph, u = _randn_params(n, True, 8)
+    psi = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(2))
+    psi /= psi.norm()
+    ho = orc.OracleHamiltonian(om, de, ph, u)
+    h = RydbergHamiltonian(om, de, ph, u, device=gpu_device)
+    st = StateVector(psi, gpu=True)
+    assert abs(float(h.expect(st)) - float(ho.expect(psi))) < 1e-12
+    ev = EnergyVariance().apply(config=None, state=st, hamiltonian=h)
+    assert abs(float(ev) - float(orc.energy_variance(ho, psi))) < 1e-11
+    m2 = EnergySecondMoment().apply(config=None, state=st, hamiltonian=h)
+    assert abs(float(m2) - float(orc.energy_second_moment(ho, psi))) < 1e-11
+    cm = CorrelationMatrix().apply(config=None, state=st, hamiltonian=h)
+    assert torch.allclose(cm, orc.correlation_matrix(psi), atol=1e-13)
+
+
+def test_reference_energy_constants(gpu_device, restore_rng_state):
+    """Seeded N=4 Hamiltonian of test_custom_callbacks.py:120-158: variance
+    3.67378968943955 (abs 4e-7) and second moment 4.2188228611101 (abs 2e-7)."""
+    from emulators_amd.emu_sv import (EnergySecondMoment, EnergyVariance, RydbergHamiltonian,
+                                      StateVector)
+    torch.manual_seed(1337)
+    n = 4
+    st = StateVector.from_state_amplitudes(eigenstates=("r", "g"),
+                                           amplitudes={"rgrg": 1.0, "grgr": 1.0})
+    om = torch.randn(n, dtype=torch.float64).to(dtype=torch.complex128)
+    de = torch.randn(n, dtype=torch.float64).to(dtype=torch.complex128)
+    u = torch.randn((n, n))
+    h = RydbergHamiltonian(om, de, torch.zeros_like(om), u, device=gpu_device)
+    ev = EnergyVariance().apply(config=None, state=st, hamiltonian=h)
+    m2 = EnergySecondMoment().apply(config=None, state=st, hamiltonian=h)
+    assert float(ev) == pytest.approx(3.67378968943955, abs=4e-7)
+    assert float(m2) == pytest.approx(4.2188228611101, abs=2e-7)
+
+
+@pytest.mark.usefixtures("restore_rng_state")
+@pytest.mark.parametrize("n,tol", [(3, 1e-8), (3, 1e-10), (5, 1e-10), (5, 1e-12)])
+def test_step_backward_vs_dense_autograd(gpu_device, n, tol):
+    """Per-step custom backward vs autograd through the dense matrix_exp
+    (test_time_evolution.py:79-103), rtol = tolerance."""
+    from emulators_amd.emu_sv.time_evolution import EvolveStateVector
+    torch.manual_seed(1337)
+    om = torch.randn(n, dtype=torch.float64, requires_grad=True)
+    de = torch.randn(n, dtype=torch.float64, requires_grad=True)
+    ph = torch.randn(n, dtype=torch.float64, requires_grad=True)
+    u0 = torch.zeros(n, n, dtype=torch.float64)
+    for i in range(n):
+        for j in range(i + 1, n):
+            u0[i, j] = u0[j, i] = torch.rand(1, dtype=torch.float64)
+    u = u0.clone().requires_grad_(True)
+    psi = torch.randn(2**n, dtype=torch.complex128)
+    psi = (psi / psi.norm()).to(gpu_device).requires_grad_(True)
+    r = torch.randn(2**n, dtype=torch.complex128)
+    r = (r * 0.71 / r.norm()).to(gpu_device)
+    out, _ = EvolveStateVector().apply(1.0, om, de, ph, u, psi, tol, [])
+    scalar = torch.vdot(r, out).real
+    grads = torch.autograd.grad(scalar, (om, de, ph, u, psi))
+
+    om2, de2, ph2 = (x.detach().clone().requires_grad_(True) for x in (om, de, ph))
+    u2 = u0.clone().requires_grad_(True)
+    psi2 = psi.detach().cpu().requires_grad_(True)
+    h = _dense_h_autograd(om2, de2, ph2, u2, n)
+    exp_scalar = torch.vdot(r.cpu(), torch.linalg.matrix_exp(-1j * h) @ psi2).real
+    expected = torch.autograd.grad(exp_scalar, (om2, de2, ph2, u2, psi2))
+    for g, eg in zip(grads, expected):
+        assert torch.allclose(g.cpu(), eg, rtol=tol, atol=tol)
+
+
+def _dense_h_autograd(om, de, ph, u, n):
+    sx = torch.tensor([[0.0, 1.0], [1.0, 0.0]], dtype=torch.complex128)
+    sy = torch.tensor([[0.0, -1j], [1j, 0.0]], dtype=torch.complex128)
+    nop = torch.tensor([[0.0, 0.0], [0.0, 1.0]], dtype=torch.complex128)
+    eye = torch.eye(2, dtype=torch.complex128)
+
+    def gate(i, g):
+        out = torch.ones(1, 1, dtype=torch.complex128)
+        for k in range(n):
+            out = torch.kron(out, g if k == i else eye)
+        return out
+
+    h = torch.zeros(2**n, 2**n, dtype=torch.complex128)
+    for i in range(n):
+        h = h + om[i] * torch.cos(ph[i]) * gate(i, sx) / 2 + om[i] * torch.sin(ph[i]) * gate(i, sy) / 2
+        h = h - de[i] * gate(i, nop)
+        for j in range(i + 1, n):
+            h = h + u[i, j] * (gate(i, nop) @ gate(j, nop))
+    return h
