@@ -70,38 +70,47 @@ def timed_run(seq, dtype, steps, warmup, tol):
     return psi, wall, KrylovStats.matvecs
 
 
-def time_hpsi(seq, dtype, reps, step):
-    """Average duration of one fused Lanczos H.psi (all passes + reductions)."""
+def time_matvec(seq, dtype, reps, step):
+    """Average duration of one Lanczos iteration's launch sequence
+    (emu_sv_lanczos_matvec: orthogonalise + all H.psi passes + reductions),
+    timed with HIP events on the stream it runs on."""
     import torch
 
     from emulators_amd import _native as native
+    from emulators_amd import autotune
     from emulators_amd.emu_sv import RydbergHamiltonian
 
     n = seq.omega.shape[1]
     ham = RydbergHamiltonian(seq.omega[step], seq.delta[step], seq.phi[step],
                              seq.interaction_matrix(0.0)[0], device="cuda")
     params = ham.device_params(torch.device("cuda"))
+    sched = autotune.best_schedule(n, dtype, torch.device("cuda"), params, ham.complex)
     g = torch.Generator(device="cuda").manual_seed(0)
-    v = torch.randn(2**n, dtype=dtype, device="cuda", generator=g)
-    prev = torch.randn(2**n, dtype=dtype, device="cuda", generator=g)
-    w = torch.empty_like(v)
+    w = torch.randn(2**n, dtype=dtype, device="cuda", generator=g)
+    pa = torch.randn(2**n, dtype=dtype, device="cuda", generator=g)
+    pb = torch.randn(2**n, dtype=dtype, device="cuda", generator=g)
+    r = torch.empty_like(w)
+
+    def call():
+        # zero coefficients keep w unchanged; the traffic is that of a real iteration
+        native.lanczos_matvec(n, params, -0.01j, w, 1.0, pa, 1.0, 0j, pb, 1.0, 0j, r,
+                              complex_coef=ham.complex, **sched.kwargs())
+
     for _ in range(3):
-        native.hpsi(n, params, -0.01j, v, 1.0, w, prev=prev, prev_scale=1.0, dots=True,
-                    complex_coef=ham.complex)
+        call()
     stream = torch.cuda.current_stream()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record(stream)
     for _ in range(reps):
-        native.hpsi(n, params, -0.01j, v, 1.0, w, prev=prev, prev_scale=1.0, dots=True,
-                    complex_coef=ham.complex)
+        call()
     stop.record(stream)
     stop.synchronize()
     ms = start.elapsed_time(stop) / reps
     bpe = 16 if dtype == torch.complex128 else 8
-    alg_bytes = 3 * (2**n) * bpe  # read v_j, read v_{j-1}, write w
-    passes = native.plan_passes(n, dtype)
-    del v, prev, w
-    return ms, alg_bytes, passes
+    alg_bytes = 3 * (2**n) * bpe  # read v_j, read v_{j-1}, write v_{j+1}
+    passes = native.plan_passes(n, dtype, sched.tile_log, sched.min_contig, kernel=sched.kernel)
+    del w, pa, pb, r
+    return ms, alg_bytes, passes, sched
 
 
 def cpu_baseline(seq, mean_iters, step, sample_iters):
@@ -159,6 +168,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c64", action="store_true")
     ap.add_argument("--duration", type=int, default=3000, help="pulse duration in ns")
+    ap.add_argument("--only-matvec", action="store_true",
+                    help="time only the Lanczos matvec (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -180,6 +191,12 @@ def main() -> int:
     n = seq.omega.shape[1]
     log(f"workload triangular26: N={n}, {nsteps} steps; timing {steps} after {args.warmup} warmup")
 
+    if args.only_matvec:
+        for dt in (torch.complex128, torch.complex64):
+            ms, nbytes, passes, sched = time_matvec(seq, dt, args.hpsi_reps, nsteps // 2)
+            log(f"{dt}: Lanczos matvec {ms:.3f} ms, {nbytes / ms / 1e6:.0f} GB/s algorithmic, "
+                f"{passes} passes, {sched}")
+        return 0
     psi128, wall128, mv128 = timed_run(seq, torch.complex128, steps, args.warmup, tol)
     log(f"c128: {steps} steps in {wall128:.2f} s, {mv128} matvecs")
     result64 = None
@@ -201,14 +218,15 @@ def main() -> int:
     torch.cuda.empty_cache()
 
     mid = min(nsteps // 2, nsteps - 1)
-    ms128, bytes128, passes128 = time_hpsi(seq, torch.complex128, args.hpsi_reps, mid)
+    ms128, bytes128, passes128, sched128 = time_matvec(seq, torch.complex128, args.hpsi_reps, mid)
     gbs128 = bytes128 / (ms128 * 1e-3) / 1e9
-    hpsi = {"c128": {"ms": ms128, "alg_bytes": bytes128, "GB/s": gbs128, "passes": passes128}}
+    hpsi = {"c128": {"ms": ms128, "alg_bytes": bytes128, "GB/s": gbs128, "passes": passes128,
+                     "schedule": sched128.kwargs()}}
     if not args.no_c64:
-        ms64, bytes64, passes64 = time_hpsi(seq, torch.complex64, args.hpsi_reps, mid)
+        ms64, bytes64, passes64, sched64 = time_matvec(seq, torch.complex64, args.hpsi_reps, mid)
         hpsi["c64"] = {"ms": ms64, "alg_bytes": bytes64, "GB/s": bytes64 / (ms64 * 1e-3) / 1e9,
-                       "passes": passes64}
-    log(f"H.psi c128 {ms128:.3f} ms ({gbs128:.0f} GB/s algorithmic)")
+                       "passes": passes64, "schedule": sched64.kwargs()}
+    log(f"Lanczos matvec c128 {ms128:.3f} ms ({gbs128:.0f} GB/s algorithmic)")
 
     mean_iters = mv128 / steps
     cpu = None
@@ -245,8 +263,10 @@ def main() -> int:
             "unit": "GB/s",
             "frac": gbs128 / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": f"fused Lanczos H.psi, complex128, {passes128} LDS-tiled passes "
-                      "(hpsi_pass_kernel) + reduction; algorithmic bytes 3*D*16 per call",
+            "kernel": f"one Lanczos iteration (emu_sv_lanczos_matvec, complex128): "
+                      f"orthogonalise + {passes128} H.psi passes ({sched128.kernel} kernel, "
+                      f"tile 2^{sched128.tile_log}) + reductions; algorithmic bytes 3*D*16 per call "
+                      "(read v_j, v_{j-1}, write v_{j+1})",
         },
         "cpu_baseline": cpu,
         "c64": result64,

@@ -23,6 +23,10 @@ EMU_SV_ENOTCONVERGED = -3
 EMU_SV_C128 = 0
 EMU_SV_C64 = 1
 EMU_SV_FLAG_COMPLEX = 1
+EMU_SV_KERNEL_DEFAULT = 0
+EMU_SV_KERNEL_LDS = 1 << 8
+EMU_SV_KERNEL_WAVE = 2 << 8
+KERNELS = {"default": EMU_SV_KERNEL_DEFAULT, "lds": EMU_SV_KERNEL_LDS, "wave": EMU_SV_KERNEL_WAVE}
 EMU_SV_MAX_QUBITS = 32
 EMU_SV_HAM_DOUBLES = 1128
 
@@ -35,16 +39,19 @@ SIGNATURES = {
     "emu_sv_last_error": (ctypes.c_char_p, []),
     "emu_sv_version": (ctypes.c_char_p, []),
     "emu_sv_pack_hamiltonian": (_i, [_i, _dp, _dp, _dp, _dp, _dp]),
-    "emu_sv_plan_passes": (_i, [_i, _i, _i, _i]),
+    "emu_sv_plan_passes": (_i, [_i, _i, _i, _i, _i]),
     "emu_sv_workspace_bytes": (ctypes.c_size_t, [_i, _i]),
     "emu_sv_hpsi": (_i, [_i, _i, _vp, _d, _d, _vp, _d, _vp, _d, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "emu_sv_lanczos_update": (_i, [_i, _i, _vp, _vp, _d, _d, _d, _vp, _d, _d, _d, _vp, _vp, _vp]),
+    "emu_sv_lanczos_matvec": (_i, [_i, _i, _vp, _d, _d, _vp, _d, _vp, _d, _d, _d, _vp, _d, _d, _d,
+                                   _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "emu_sv_lincomb": (_i, [_i, _i, _i, ctypes.POINTER(_vp), _dp, _vp, _vp]),
     "emu_sv_dot": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp]),
     "emu_sv_scale": (_i, [_i, _i, _vp, _d, _vp]),
     "emu_sv_convert": (_i, [_i, _i, _vp, _i, _vp, _vp]),
     "emu_sv_correlation": (_i, [_i, _i, _vp, _vp, _vp, _vp]),
     "emu_sv_expm": (_i, [_i, _dp, _dp]),
+    "emu_sv_lanczos_exp": (_i, [_i, _dp, _dp, _d, _d, _d, _i, _dp]),
 }
 
 
@@ -133,8 +140,9 @@ def pack_hamiltonian(omega, delta, phi, interaction) -> tuple[np.ndarray, bool]:
     return block, bool(np.any(ph != 0.0))
 
 
-def plan_passes(n: int, dtype: torch.dtype, tile_log: int = 0, min_contig: int = 0) -> int:
-    rc = LIB.emu_sv_plan_passes(n, dtype_code(dtype), tile_log, min_contig)
+def plan_passes(n: int, dtype: torch.dtype, tile_log: int = 0, min_contig: int = 0,
+                kernel: str = "default") -> int:
+    rc = LIB.emu_sv_plan_passes(n, dtype_code(dtype), tile_log, min_contig, KERNELS[kernel])
     if rc < 0:
         check(rc, "plan_passes")
     return rc
@@ -147,7 +155,8 @@ class Workspace:
 
     def __init__(self, device: torch.device, n: int):
         nbytes = LIB.emu_sv_workspace_bytes(n, 0)
-        self.scratch = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        # zero-filled: holds the reduction ticket counter, which the library re-arms
+        self.scratch = torch.zeros(nbytes, dtype=torch.uint8, device=device)
         self.results = torch.zeros(64, dtype=torch.float64, device=device)
         self.n = n
 
@@ -164,7 +173,7 @@ class Workspace:
 def hpsi(n: int, ham_dev: torch.Tensor, sigma: complex, v: torch.Tensor, v_scale: float,
          w: torch.Tensor, prev: torch.Tensor | None = None, prev_scale: float = 1.0,
          dots: bool = False, complex_coef: bool = True, tile_log: int = 0,
-         min_contig: int = 0) -> torch.Tensor | None:
+         min_contig: int = 0, kernel: str = "default") -> torch.Tensor | None:
     """w = sigma * H (v_scale v); optionally returns the 7 fused dots (device)."""
     _require_device_vec(v, "hpsi")
     _require_device_vec(w, "hpsi")
@@ -174,11 +183,33 @@ def hpsi(n: int, ham_dev: torch.Tensor, sigma: complex, v: torch.Tensor, v_scale
         raise ValueError("hpsi: prev must match v")
     ws = Workspace.get(v.device, n)
     out = ws.results[0:7] if dots else None
-    flags = EMU_SV_FLAG_COMPLEX if complex_coef else 0
+    flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | KERNELS[kernel]
     check(LIB.emu_sv_hpsi(n, dtype_code(v.dtype), _ptr(ham_dev), float(sigma.real),
                           float(sigma.imag), _ptr(v), float(v_scale), _ptr(prev),
                           float(prev_scale), _ptr(w), _ptr(out), _ptr(ws.scratch), tile_log,
                           min_contig, flags, _stream(v.device)), "hpsi")
+    return out
+
+
+def lanczos_matvec(n: int, ham_dev: torch.Tensor, sigma: complex, w: torch.Tensor, w_scale: float,
+                   pa: torch.Tensor | None, pa_scale: float, a: complex, pb: torch.Tensor,
+                   pb_scale: float, b: complex, r: torch.Tensor, complex_coef: bool = True,
+                   tile_log: int = 0, min_contig: int = 0, kernel: str = "default") -> torch.Tensor:
+    """z = w_scale w - a pa_scale pa - b pb_scale pb (into w); r = sigma H z.
+    Returns a device slice [||z||^2, 7 dots of (z, r, prev=pb)]."""
+    for t in (w, pb, r) + ((pa,) if pa is not None else ()):
+        _require_device_vec(t, "lanczos_matvec")
+        if t.dtype != w.dtype or t.numel() != (1 << n):
+            raise ValueError("lanczos_matvec: vectors must be 2**n elements of one dtype")
+    ws = Workspace.get(w.device, n)
+    out = ws.results[24:32]
+    flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | KERNELS[kernel]
+    check(LIB.emu_sv_lanczos_matvec(n, dtype_code(w.dtype), _ptr(ham_dev), float(sigma.real),
+                                    float(sigma.imag), _ptr(w), float(w_scale), _ptr(pa),
+                                    float(pa_scale), float(a.real), float(a.imag), _ptr(pb),
+                                    float(pb_scale), float(b.real), float(b.imag), _ptr(r),
+                                    _ptr(out[0:1]), _ptr(out[1:8]), _ptr(ws.scratch), tile_log,
+                                    min_contig, flags, _stream(w.device)), "lanczos_matvec")
     return out
 
 
@@ -256,3 +287,19 @@ def expm(a: np.ndarray) -> np.ndarray:
     check(LIB.emu_sv_expm(m, a.view(np.float64).ctypes.data_as(dp),
                           e.view(np.float64).ctypes.data_as(dp)), "expm")
     return e
+
+
+def lanczos_exp(T: np.ndarray, m: int, sigma: complex, n2: float, want_coeffs: bool):
+    """(|expd[m,0]|, |expd[m+1,0]|, coeffs or None) for the leading m x m block
+    of a Hermitian Lanczos matrix T (see emu_sv_lanczos_exp)."""
+    dp = ctypes.POINTER(ctypes.c_double)
+    diag = np.ascontiguousarray(np.diagonal(T)[:m]).astype(np.complex128)
+    sub = np.ascontiguousarray(np.real(np.diagonal(T, -1)[: max(m - 1, 0)])).astype(np.float64)
+    if sub.size == 0:
+        sub = np.zeros(1)
+    out = np.empty(2 + 2 * (m + 1) if want_coeffs else 2, dtype=np.float64)
+    check(LIB.emu_sv_lanczos_exp(m, diag.view(np.float64).ctypes.data_as(dp), sub.ctypes.data_as(dp),
+                                 float(sigma.real), float(sigma.imag), float(n2), int(want_coeffs),
+                                 out.ctypes.data_as(dp)), "lanczos_exp")
+    coeffs = out[2:].view(np.complex128) if want_coeffs else None
+    return out[0], out[1], coeffs

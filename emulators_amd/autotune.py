@@ -1,0 +1,112 @@
+"""Schedule autotuner for the H.psi passes.
+
+A schedule is (kernel kind, tile_log, min_contig):
+
+* "wave": a wavefront owns 2^tile_log amplitudes (tile_log = 6 lane bits +
+  3..5 register bits); flips are register or cross-lane operations.
+* "lds":  a 256-thread workgroup stages 2^tile_log amplitudes in LDS.
+
+``tile_log`` and ``min_contig`` fix the qubit split: pass 1 covers bits
+[0, tile_log); each later pass covers the next tile_log - c high bits plus c
+contiguous low bits (c >= min_contig) for coalesced segments.
+
+The first use of a (device, n, dtype, coefficient kind) with n >= 16 times
+every candidate on the fused Lanczos H.psi (two untimed and three timed
+calls each, HIP events on the current stream) and keeps the fastest for the
+rest of the process.  Set EMU_SV_AUTOTUNE=0 to use the library default.
+"""
+
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass
+
+import torch
+
+from emulators_amd import _native as native
+
+MIN_TUNED_QUBITS = 16
+
+
+@dataclass(frozen=True)
+class Schedule:
+    kernel: str = "default"
+    tile_log: int = 0
+    min_contig: int = 0
+
+    def kwargs(self) -> dict:
+        return {"kernel": self.kernel, "tile_log": self.tile_log, "min_contig": self.min_contig}
+
+
+DEFAULT = Schedule()
+
+CANDIDATES = {
+    torch.complex64: [Schedule("wave", 9, 2), Schedule("wave", 10, 2), Schedule("wave", 10, 3),
+                      Schedule("wave", 11, 2), Schedule("wave", 11, 3), Schedule("wave", 11, 4),
+                      Schedule("lds", 11, 3), Schedule("lds", 12, 3), Schedule("lds", 13, 3),
+                      Schedule("lds", 14, 2)],
+    torch.complex128: [Schedule("wave", 9, 2), Schedule("wave", 10, 2), Schedule("wave", 10, 3),
+                       Schedule("wave", 11, 2), Schedule("wave", 11, 3), Schedule("lds", 10, 2),
+                       Schedule("lds", 11, 2), Schedule("lds", 12, 2), Schedule("lds", 13, 2)],
+}
+
+_cache: dict = {}
+_timings: dict = {}
+
+
+def _enabled() -> bool:
+    return os.environ.get("EMU_SV_AUTOTUNE", "1") != "0"
+
+
+def time_schedule(n: int, dtype: torch.dtype, params: torch.Tensor, sched: Schedule,
+                  complex_coef: bool, vecs, warmup: int = 2, reps: int = 3) -> float:
+    """Average ms of one fused Lanczos H.psi (with prev and reductions)."""
+    v, prev, w = vecs
+    for _ in range(warmup):
+        native.hpsi(n, params, -0.01j, v, 1.0, w, prev=prev, prev_scale=1.0, dots=True,
+                    complex_coef=complex_coef, **sched.kwargs())
+    stream = torch.cuda.current_stream(v.device)
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(reps):
+        native.hpsi(n, params, -0.01j, v, 1.0, w, prev=prev, prev_scale=1.0, dots=True,
+                    complex_coef=complex_coef, **sched.kwargs())
+    stop.record(stream)
+    stop.synchronize()
+    return start.elapsed_time(stop) / reps
+
+
+def tune(n: int, dtype: torch.dtype, device: torch.device, params: torch.Tensor,
+         complex_coef: bool, candidates=None) -> tuple[Schedule, dict]:
+    """Time every candidate; returns (best, {schedule: ms})."""
+    g = torch.Generator(device=device).manual_seed(1234)
+    v = torch.randn(2**n, dtype=dtype, device=device, generator=g)
+    prev = torch.randn(2**n, dtype=dtype, device=device, generator=g)
+    w = torch.empty_like(v)
+    times = {}
+    for sched in candidates or CANDIDATES[dtype]:
+        times[sched] = time_schedule(n, dtype, params, sched, complex_coef, (v, prev, w))
+    best = min(times, key=times.get)
+    del v, prev, w
+    return best, times
+
+
+def best_schedule(n: int, dtype: torch.dtype, device: torch.device, params: torch.Tensor,
+                  complex_coef: bool) -> Schedule:
+    """The tuned schedule for this shape (DEFAULT below MIN_TUNED_QUBITS or when disabled)."""
+    if n < MIN_TUNED_QUBITS or not _enabled():
+        return DEFAULT
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    key = (idx, n, dtype, bool(complex_coef))
+    sched = _cache.get(key)
+    if sched is None:
+        sched, times = tune(n, dtype, device, params, complex_coef)
+        _cache[key] = sched
+        _timings[key] = times
+    return sched
+
+
+def tuned_table() -> dict:
+    """{(device, n, dtype, complex): {schedule: ms}} of every tuning done so far."""
+    return dict(_timings)

@@ -135,3 +135,136 @@ extern "C" int emu_sv_expm(int m, const double *a_in, double *e_out) {
   }
   return EMU_SV_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Lanczos exponential from the symmetric tridiagonal form.
+//
+// For op = sigma*H with H Hermitian, the Lanczos matrix T (T[k,k] = sigma
+// alpha_k, T[k+1,k] = |sigma| beta_k) satisfies T = W^-1 (sigma J) W with J
+// the real symmetric tridiagonal (alpha, beta) and W = diag(w^k), w =
+// sigma/|sigma|.  With J = Z diag(lambda) Z^T:
+//   (exp(T) e_0)_k            = w^-k     sum_i Z[k,i] exp(sigma lambda_i) Z[0,i]
+//   e_{m-1}^T phi_p(T) e_0    = w^-(m-1) sum_i Z[m-1,i] phi_p(sigma lambda_i) Z[0,i]
+// where phi_1(z) = (e^z - 1)/z and phi_2(z) = (e^z - 1 - z)/z^2 give the two
+// extra entries of exp of the Expokit extended matrix (Sidje 1998).  The
+// eigen-decomposition is the implicit symmetric QR iteration with Wilkinson
+// shifts (Golub & Van Loan, Matrix Computations, section 8.3); only the rows
+// of Z that are needed are accumulated: rows 0 and m-1 per iteration, all
+// rows when the combination coefficients are requested.
+// ---------------------------------------------------------------------------
+namespace {
+
+bool tridiag_eig(int n, double *d, double *e, int nrow, const int *rows, double *zr) {
+  for (int i = 0; i < nrow; ++i)
+    for (int j = 0; j < n; ++j) zr[(size_t)i * n + j] = (rows[i] == j) ? 1.0 : 0.0;
+  const double eps = 2.220446049250313e-16;
+  int hi = n - 1, iters = 0;
+  while (hi > 0) {
+    if (std::fabs(e[hi - 1]) <= eps * (std::fabs(d[hi - 1]) + std::fabs(d[hi]))) {
+      e[hi - 1] = 0.0;
+      --hi;
+      continue;
+    }
+    int lo = hi - 1;
+    while (lo > 0 && std::fabs(e[lo - 1]) > eps * (std::fabs(d[lo - 1]) + std::fabs(d[lo]))) --lo;
+    if (lo > 0) e[lo - 1] = 0.0;
+    if (++iters > 100 * n) return false;
+    // Wilkinson shift: eigenvalue of the trailing 2x2 closer to d[hi].
+    const double dd = 0.5 * (d[hi - 1] - d[hi]);
+    const double eh = e[hi - 1];
+    const double mu = d[hi] - eh * eh / (dd + std::copysign(std::hypot(dd, eh), dd));
+    double x = d[lo] - mu, z = e[lo], bulge = 0.0;
+    for (int k = lo; k < hi; ++k) {
+      const double r = std::hypot(x, z);
+      const double c = r == 0.0 ? 1.0 : x / r, s = r == 0.0 ? 0.0 : z / r;
+      if (k > lo) e[k - 1] = r;
+      const double a = d[k], b = e[k], d1 = d[k + 1];
+      d[k] = c * c * a + 2.0 * c * s * b + s * s * d1;
+      d[k + 1] = s * s * a - 2.0 * c * s * b + c * c * d1;
+      e[k] = c * s * (d1 - a) + (c * c - s * s) * b;
+      if (k + 1 < hi) {
+        bulge = s * e[k + 1];
+        e[k + 1] = c * e[k + 1];
+      }
+      for (int i = 0; i < nrow; ++i) {
+        double *row = zr + (size_t)i * n;
+        const double zk = row[k], zk1 = row[k + 1];
+        row[k] = c * zk + s * zk1;
+        row[k + 1] = -s * zk + c * zk1;
+      }
+      x = e[k];
+      z = bulge;
+    }
+  }
+  return true;
+}
+
+// phi_p(z) for p = 1, 2: series near 0, closed form elsewhere.
+cd phi(int p, cd z) {
+  if (std::abs(z) < 0.5) {
+    // sum_k z^k / (k + p)!
+    cd term = 1.0, sum = 0.0;
+    double fact = p == 1 ? 1.0 : 2.0;  // p!
+    for (int k = 0; k < 30; ++k) {
+      sum += term / fact;
+      term *= z;
+      fact *= (k + p + 1);
+    }
+    return sum;
+  }
+  const cd ez = std::exp(z);
+  return p == 1 ? (ez - 1.0) / z : (ez - 1.0 - z) / (z * z);
+}
+
+}  // namespace
+
+extern "C" int emu_sv_lanczos_exp(int m, const double *tdiag, const double *tsub, double sigma_re,
+                                  double sigma_im, double n2, int want_coeffs, double *out) {
+  if (m < 1 || m > 4096 || !tdiag || (m > 1 && !tsub) || !out) return EMU_SV_EINVAL;
+  const cd sigma(sigma_re, sigma_im);
+  const double asig = std::abs(sigma);
+  if (asig == 0.0) return EMU_SV_EINVAL;
+  const cd w = sigma / asig;
+  std::vector<double> d(m), e(m > 1 ? m - 1 : 1, 0.0);
+  for (int k = 0; k < m; ++k) d[k] = (cd(tdiag[2 * k], tdiag[2 * k + 1]) / sigma).real();
+  for (int k = 0; k + 1 < m; ++k) e[k] = tsub[k] / asig;
+  std::vector<int> rows;
+  if (want_coeffs) {
+    for (int k = 0; k < m; ++k) rows.push_back(k);
+  } else {
+    rows.push_back(0);
+    if (m > 1) rows.push_back(m - 1);
+  }
+  const int nrow = (int)rows.size();
+  std::vector<double> zr((size_t)nrow * m);
+  if (!tridiag_eig(m, d.data(), e.data(), nrow, rows.data(), zr.data())) return EMU_SV_EINVAL;
+  const double *z0 = zr.data();
+  const double *zl = zr.data() + (size_t)(nrow - 1) * m;  // row m-1 (or row 0 when m == 1)
+  cd s1 = 0.0, s2 = 0.0;
+  for (int i = 0; i < m; ++i) {
+    const cd x = sigma * d[i];
+    s1 += zl[i] * phi(1, x) * z0[i];
+    s2 += zl[i] * phi(2, x) * z0[i];
+  }
+  const cd wl = std::pow(std::conj(w), m - 1);  // w^-(m-1), |w| = 1
+  const cd c_ext = n2 * wl * s1;
+  out[0] = std::abs(c_ext);
+  out[1] = std::abs(n2 * wl * s2);
+  if (want_coeffs) {
+    std::vector<cd> ex(m);
+    for (int i = 0; i < m; ++i) ex[i] = std::exp(sigma * d[i]) * z0[i];
+    cd wk = 1.0;
+    for (int k = 0; k < m; ++k) {
+      cd c = 0.0;
+      const double *zk = zr.data() + (size_t)k * m;
+      for (int i = 0; i < m; ++i) c += zk[i] * ex[i];
+      c *= wk;
+      out[2 + 2 * k] = c.real();
+      out[3 + 2 * k] = c.imag();
+      wk *= std::conj(w);
+    }
+    out[2 + 2 * m] = c_ext.real();
+    out[3 + 2 * m] = c_ext.imag();
+  }
+  return EMU_SV_OK;
+}

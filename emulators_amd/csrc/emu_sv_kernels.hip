@@ -55,10 +55,15 @@ constexpr int kWaves = kThreads / 64;
 constexpr int kMaxTileLog = 14;   // LDS tile <= 2^14 elements
 constexpr int kThreadBits = 8;    // log2(kThreads)
 constexpr int kNumDots = 8;
-constexpr int kStreamBlocks = 1024;  // grid of the streaming kernels
+constexpr int kDiagRow = 16;  // E_out, a_0 .. a_{t-1} of one pass-1 tile
+constexpr int kStreamBlocks = 2048;  // grid of the streaming kernels
 constexpr int kDefaultTileLogC64 = 13;
 constexpr int kDefaultTileLogC128 = 12;
 constexpr int kDefaultContigC64 = 3;
+constexpr int kDefaultWaveLogC64 = 11;
+constexpr int kDefaultWaveLogC128 = 10;
+constexpr int kKernelLds = 1;
+constexpr int kKernelWave = 2;
 constexpr int kDefaultContigC128 = 2;
 
 // Offsets into the packed parameter block (see emu_sv_hip.h).
@@ -96,6 +101,15 @@ struct PassArgs {
   void *out;          // output (read-modify-write when accumulate)
   const double *ham;  // packed parameter block (device)
   double *partials;   // per-workgroup partial sums (kNumDots each) or nullptr
+  const double *dtab; // per-tile diagonal constants of pass 1 (kDiagRow doubles per tile)
+  // Orthogonalising load (pass 1 of a fused Lanczos matvec): the tile input
+  // is z = v_scale*v - ca*pa - cb*pb, written back to v (zout) and reduced to
+  // ||z||^2 in partials2.
+  int ortho;
+  void *zout;
+  const void *pa, *pb;
+  double ca_re, ca_im, cb_re, cb_im;
+  double *partials2;
   double v_scale, prev_scale;
   double sig_re, sig_im;
   int n;               // qubits
@@ -105,6 +119,39 @@ struct PassArgs {
   int accumulate;      // out += ...
   int xcd;             // XCD-aware tile order
 };
+
+// Load one input amplitude of a pass: v_scale*v[x], or in orthogonalising
+// mode z = v_scale*v[x] - ca*pa[x] - cb*pb[x] (FP64 arithmetic, stored back
+// to v and accumulated into nrm).
+template <typename T>
+__device__ __forceinline__ typename Vec2<T>::type load_input(const PassArgs &a, uint64_t x,
+                                                             double &nrm) {
+  using T2 = typename Vec2<T>::type;
+  const T2 v = reinterpret_cast<const T2 *>(a.v)[x];
+  T2 z;
+  if (!a.ortho) {
+    const T vs = static_cast<T>(a.v_scale);
+    z.x = v.x * vs;
+    z.y = v.y * vs;
+    return z;
+  }
+  double zr = a.v_scale * static_cast<double>(v.x), zi = a.v_scale * static_cast<double>(v.y);
+  if (a.pa) {
+    const T2 p = reinterpret_cast<const T2 *>(a.pa)[x];
+    zr -= a.ca_re * p.x - a.ca_im * p.y;
+    zi -= a.ca_re * p.y + a.ca_im * p.x;
+  }
+  if (a.pb) {
+    const T2 p = reinterpret_cast<const T2 *>(a.pb)[x];
+    zr -= a.cb_re * p.x - a.cb_im * p.y;
+    zi -= a.cb_re * p.y + a.cb_im * p.x;
+  }
+  z.x = static_cast<T>(zr);
+  z.y = static_cast<T>(zi);
+  reinterpret_cast<T2 *>(a.zout)[x] = z;
+  nrm += static_cast<double>(z.x) * z.x + static_cast<double>(z.y) * z.y;
+  return z;
+}
 
 // ---------------------------------------------------------------------------
 // H.psi pass kernel.  CPLX: complex flip coefficients (some phi != 0).
@@ -137,44 +184,15 @@ __global__ __launch_bounds__(kThreads) void hpsi_pass_kernel(const PassArgs a) {
   auto gbit = [&](int i) -> int { return i < c ? i : s + (i - c); };
 
   const double *__restrict__ H = a.ham;
-  const T2 *__restrict__ vin = reinterpret_cast<const T2 *>(a.v);
-  const T vs = static_cast<T>(a.v_scale);
 
   // Stage the tile.
-  for (uint32_t l = tid; l < tsize; l += kThreads) {
-    T2 z = vin[gidx(l)];
-    z.x *= vs;
-    z.y *= vs;
-    tile[l] = z;
-  }
+  double onrm[1] = {0.0};
+  for (uint32_t l = tid; l < tsize; l += kThreads) tile[l] = load_input<T>(a, gidx(l), onrm[0]);
   if (a.diag) {
-    if (tid < t) {
-      const int b = gbit(tid);
-      double acc = -H[kDelta + b];
-      uint64_t bm = base;
-      while (bm) {
-        const int q = __ffsll((unsigned long long)bm) - 1;
-        bm &= bm - 1;
-        acc += H[kU + b * 32 + q];
-      }
-      s_ap[tid] = acc;
-    }
-    if (tid == 63) {
-      double e = 0.0;
-      uint64_t bm = base;
-      while (bm) {
-        const int q = __ffsll((unsigned long long)bm) - 1;
-        bm &= bm - 1;
-        e -= H[kDelta + q];
-        uint64_t bm2 = bm;
-        while (bm2) {
-          const int q2 = __ffsll((unsigned long long)bm2) - 1;
-          bm2 &= bm2 - 1;
-          e += H[kU + q * 32 + q2];
-        }
-      }
-      s_eout = e;
-    }
+    // Outer-bit constants of this tile, tabulated by diag_prep_kernel.
+    const double *row = a.dtab + g * kDiagRow;
+    if (tid < t) s_ap[tid] = row[1 + tid];
+    if (tid == 63) s_eout = row[0];
     for (int e = tid; e < t * t; e += kThreads) {
       const int i = e / t, j = e % t;
       s_U[i][j] = H[kU + gbit(i) * 32 + gbit(j)];
@@ -293,6 +311,291 @@ __global__ __launch_bounds__(kThreads) void hpsi_pass_kernel(const PassArgs a) {
     }
   }
   if (DOTS) block_reduce_store<7>(dots, s_red, a.partials + (uint64_t)blockIdx.x * kNumDots);
+  if (a.ortho) {
+    __syncthreads();
+    block_reduce_store<1>(onrm, s_red, a.partials2 + (uint64_t)blockIdx.x * kNumDots);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Wave-tile H.psi pass kernel.  One wavefront owns a tile of 2^(6+KREG)
+// amplitudes: local bits 0..5 are the lane index, local bits 6..6+KREG-1
+// index KREG-bit register slots.  A flip of a register bit reads another
+// register of the same lane; a flip of a lane bit is a cross-lane exchange
+// (DPP quad_perm / row mirrors for xor 1,2,4,8, v_permlane16/32_swap for xor
+// 16, 32).  No LDS, no barrier: occupancy is set by registers alone.
+// ---------------------------------------------------------------------------
+template <int MASK>
+__device__ __forceinline__ int lane_xor(int v, int lane) {
+  if constexpr (MASK == 1) {
+    return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
+  } else if constexpr (MASK == 2) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+  } else if constexpr (MASK == 4) {
+    const int t = __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    return __builtin_amdgcn_update_dpp(0, t, 0x1B, 0xF, 0xF, false);         // quad_perm 3,2,1,0
+  } else if constexpr (MASK == 8) {
+    const int t = __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);  // row_mirror
+    return __builtin_amdgcn_update_dpp(0, t, 0x141, 0xF, 0xF, false);        // row_half_mirror
+  } else if constexpr (MASK == 16) {
+    const auto p = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
+    return (lane & 16) ? (int)p[0] : (int)p[1];
+  } else {
+    const auto p = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+    return (lane & 32) ? (int)p[0] : (int)p[1];
+  }
+}
+
+template <int MASK>
+__device__ __forceinline__ float lane_xor(float v, int lane) {
+  return __int_as_float(lane_xor<MASK>(__float_as_int(v), lane));
+}
+
+template <int MASK>
+__device__ __forceinline__ double lane_xor(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = lane_xor<MASK>((int)(b & 0xffffffffll), lane);
+  const int hi = lane_xor<MASK>((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+template <typename T, int P>
+__device__ __forceinline__ void lane_flip_add(const typename Vec2<T>::type psi, int lane,
+                                              uint32_t fm, T cr, T cis, T &hr, T &hi) {
+  if ((fm >> P) & 1u) {
+    const T nx = lane_xor<1 << P>(psi.x, lane);
+    const T ny = lane_xor<1 << P>(psi.y, lane);
+    hr += cr * nx - cis * ny;
+    hi += cr * ny + cis * nx;
+  }
+}
+
+template <typename T, int KREG, bool CPLX, bool DOTS>
+__global__ __launch_bounds__(kThreads) void hpsi_wave_kernel(const PassArgs a) {
+  using T2 = typename Vec2<T>::type;
+  constexpr int R = 1 << KREG;
+  constexpr int t = 6 + KREG;
+  const int lane = threadIdx.x & 63;
+  const int c = a.c, s = a.s, m = a.m, n = a.n;
+  const uint64_t ntiles = 1ull << (n - t);
+  uint64_t blk = blockIdx.x;
+  const uint64_t nblk = ntiles >= (uint64_t)kWaves ? ntiles / kWaves : 1;
+  if (a.xcd && nblk >= 8) blk = (blk & 7) * (nblk >> 3) + (blk >> 3);
+  const uint64_t g = blk * kWaves + (threadIdx.x >> 6);
+  if (g >= ntiles) return;  // whole wave exits (small problems only)
+  const int nlo = s - c;
+  const uint64_t base = ((g & ((1ull << nlo) - 1)) << c) | ((g >> nlo) << (s + m));
+  const uint32_t cmask = (1u << c) - 1;
+  auto gidx = [&](uint32_t l) -> uint64_t {
+    return base | (uint64_t)(l & cmask) | ((uint64_t)(l >> c) << s);
+  };
+  auto gbit = [&](int i) -> int { return i < c ? i : s + (i - c); };
+  const double *__restrict__ H = a.ham;
+  T2 *__restrict__ out = reinterpret_cast<T2 *>(a.out);
+  const T2 *__restrict__ prevp = reinterpret_cast<const T2 *>(a.prev);
+
+  // Load the tile: slot r of lane holds local index lane | r << 6.
+  T2 psi[R];
+  double onrm = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) psi[r] = load_input<T>(a, gidx((uint32_t)lane | ((uint32_t)r << 6)), onrm);
+
+  // Flip coefficients per local bit (uniform), conj for a clear bit.
+  uint32_t fm = 0;
+  T cre[t], cim[t];
+#pragma unroll
+  for (int i = 0; i < t; ++i) {
+    const int b = gbit(i);
+    const bool on = (a.flip_mask >> b) & 1u;
+    fm |= on ? (1u << i) : 0u;
+    cre[i] = on ? static_cast<T>(H[kCre + b]) : T(0);
+    cim[i] = on ? static_cast<T>(H[kCim + b]) : T(0);
+  }
+
+  // Diagonal: E(lane, r) = dbase(lane) + sum_{j in r} G_j(lane) + sum_{j<j' in r} U_jj'.
+  double dbase = 0.0, Gj[KREG], Ur[KREG][KREG];
+  if (a.diag) {
+    // Outer-bit constants of this tile, tabulated by diag_prep_kernel.
+    const double *row = a.dtab + g * kDiagRow;
+    const double ai = lane < t ? row[1 + lane] : 0.0;
+    const double eout = row[0];
+    dbase = eout;
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+      const double ap = __shfl(ai, p, 64);
+      if ((lane >> p) & 1) {
+        dbase += ap;
+#pragma unroll
+        for (int p2 = 0; p2 < p; ++p2)
+          if ((lane >> p2) & 1) dbase += H[kU + gbit(p) * 32 + gbit(p2)];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < KREG; ++j) {
+      double gj = __shfl(ai, 6 + j, 64);
+#pragma unroll
+      for (int p = 0; p < 6; ++p)
+        if ((lane >> p) & 1) gj += H[kU + gbit(6 + j) * 32 + gbit(p)];
+      Gj[j] = gj;
+#pragma unroll
+      for (int j2 = 0; j2 < KREG; ++j2) Ur[j][j2] = H[kU + gbit(6 + j) * 32 + gbit(6 + j2)];
+    }
+  }
+
+  // Per-lane signed imaginary parts for the lane-bit flips.
+  T cis[6];
+#pragma unroll
+  for (int p = 0; p < 6; ++p) cis[p] = ((lane >> p) & 1) ? cim[p] : -cim[p];
+
+  const T sr = static_cast<T>(a.sig_re), si = static_cast<T>(a.sig_im);
+  const T ps = static_cast<T>(a.prev_scale);
+  double dots[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const T2 p0 = psi[r];
+    T hr = 0, hi = 0;
+    // register-bit flips (compile-time slots)
+#pragma unroll
+    for (int j = 0; j < KREG; ++j) {
+      if ((fm >> (6 + j)) & 1u) {
+        const T2 nb = psi[r ^ (1 << j)];
+        const T ci = CPLX ? (((r >> j) & 1) ? cim[6 + j] : -cim[6 + j]) : T(0);
+        hr += cre[6 + j] * nb.x - ci * nb.y;
+        hi += cre[6 + j] * nb.y + ci * nb.x;
+      }
+    }
+    // lane-bit flips
+    lane_flip_add<T, 0>(p0, lane, fm, cre[0], CPLX ? cis[0] : T(0), hr, hi);
+    lane_flip_add<T, 1>(p0, lane, fm, cre[1], CPLX ? cis[1] : T(0), hr, hi);
+    lane_flip_add<T, 2>(p0, lane, fm, cre[2], CPLX ? cis[2] : T(0), hr, hi);
+    lane_flip_add<T, 3>(p0, lane, fm, cre[3], CPLX ? cis[3] : T(0), hr, hi);
+    lane_flip_add<T, 4>(p0, lane, fm, cre[4], CPLX ? cis[4] : T(0), hr, hi);
+    lane_flip_add<T, 5>(p0, lane, fm, cre[5], CPLX ? cis[5] : T(0), hr, hi);
+    if (a.diag) {
+      double e = dbase;
+#pragma unroll
+      for (int j = 0; j < KREG; ++j) {
+        if ((r >> j) & 1) {
+          e += Gj[j];
+#pragma unroll
+          for (int j2 = 0; j2 < j; ++j2)
+            if ((r >> j2) & 1) e += Ur[j][j2];
+        }
+      }
+      hr += static_cast<T>(e * static_cast<double>(p0.x));
+      hi += static_cast<T>(e * static_cast<double>(p0.y));
+    }
+    T2 w;
+    w.x = sr * hr - si * hi;
+    w.y = sr * hi + si * hr;
+    const uint64_t x = gidx((uint32_t)lane | ((uint32_t)r << 6));
+    if (a.accumulate) {
+      const T2 o = out[x];
+      w.x += o.x;
+      w.y += o.y;
+    }
+    out[x] = w;
+    if (DOTS) {
+      const double wr = w.x, wi = w.y, vr = p0.x, vi = p0.y;
+      dots[0] += wr * wr + wi * wi;
+      dots[1] += vr * wr + vi * wi;
+      dots[2] += vr * wi - vi * wr;
+      if (prevp) {
+        const T2 q = prevp[x];
+        const double pr = static_cast<double>(q.x * ps), pi = static_cast<double>(q.y * ps);
+        dots[3] += pr * wr + pi * wi;
+        dots[4] += pr * wi - pi * wr;
+        dots[5] += vr * pr + vi * pi;
+        dots[6] += vr * pi - vi * pr;
+      }
+    }
+  }
+  if (DOTS) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const double sm = wave_sum(dots[k]);
+      if (lane == 0) a.partials[g * kNumDots + k] = sm;
+    }
+  }
+  if (a.ortho) {
+    const double sm = wave_sum(onrm);
+    if (lane == 0) a.partials2[g * kNumDots] = sm;
+  }
+}
+
+// Per-tile constants of the diagonal for a pass-1 tile [0, t): with h the
+// outer bits (x >> t), E_out(h) = sum_{q in h} -d_q + sum_{q<q' in h} U_qq'
+// and a_p(h) = -d_p + sum_{q in h} U_pq for the t tile bits.  One thread per
+// tile; every load is independent, so the loops pipeline.
+__global__ __launch_bounds__(kThreads) void diag_prep_kernel(int n, int t, const double *H,
+                                                             double *table) {
+  const uint64_t ntiles = 1ull << (n - t);
+  const uint64_t h = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (h >= ntiles) return;
+  const uint64_t outer = h << t;
+  double row[kDiagRow];
+#pragma unroll
+  for (int p = 0; p < kDiagRow; ++p) row[p] = 0.0;
+  double e = 0.0;
+  for (int q = t; q < n; ++q) {
+    if (!((outer >> q) & 1)) continue;
+    e -= H[kDelta + q];
+    for (int q2 = q + 1; q2 < n; ++q2)
+      if ((outer >> q2) & 1) e += H[kU + q * 32 + q2];
+  }
+  row[0] = e;
+#pragma unroll
+  for (int p = 0; p < kDiagRow - 1; ++p) {
+    if (p < t) {
+      double ap = -H[kDelta + p];
+      for (int q = t; q < n; ++q)
+        if ((outer >> q) & 1) ap += H[kU + p * 32 + q];
+      row[1 + p] = ap;
+    }
+  }
+  double *out = table + h * kDiagRow;
+#pragma unroll
+  for (int p = 0; p < kDiagRow; ++p) out[p] = row[p];
+}
+
+// Two-stage deterministic reduction of nrow rows of kNumDots partial sums:
+// every block sums a fixed contiguous chunk, the last block to finish (ticket
+// counter) adds the block sums in block order, writes K results and re-arms
+// the counter.
+constexpr int kFinalizeBlocks = 128;
+__global__ __launch_bounds__(kThreads) void finalize2_kernel(const double *rows, int nrow, int K,
+                                                             double *stage, unsigned *ticket,
+                                                             double *out) {
+  __shared__ double red[kThreads];
+  __shared__ bool last;
+  const int per = (nrow + gridDim.x - 1) / gridDim.x;
+  const int r0 = blockIdx.x * per;
+  const int r1 = min(nrow, r0 + per);
+  // thread = (row lane, value) with 32 row lanes x 8 values
+  const int v = threadIdx.x & 7, rl = threadIdx.x >> 3;
+  double acc = 0.0;
+  for (int r = r0 + rl; r < r1; r += kThreads / 8) acc += rows[(uint64_t)r * kNumDots + v];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int hlf = kThreads / 2; hlf >= 8; hlf >>= 1) {
+    if (threadIdx.x < hlf) red[threadIdx.x] += red[threadIdx.x + hlf];
+    __syncthreads();
+  }
+  if (threadIdx.x < kNumDots) stage[blockIdx.x * kNumDots + threadIdx.x] = red[threadIdx.x];
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (threadIdx.x < K) {
+    double sm = 0.0;
+    for (unsigned b = 0; b < gridDim.x; ++b)
+      sm += __hip_atomic_load(&stage[b * kNumDots + threadIdx.x], __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+    out[threadIdx.x] = sm;
+  }
+  if (threadIdx.x == 0) atomicExch(ticket, 0u);
 }
 
 // Sum nblk rows of K partials in a fixed order (deterministic).
@@ -313,68 +616,123 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(const double *partia
   }
 }
 
+// Streaming kernels work on 16-byte packets: one complex128 or two complex64
+// amplitudes per lane and load.
+template <typename T> struct Packet;
+template <> struct Packet<float> {
+  using type = float4;
+  static constexpr int amps = 2;
+};
+template <> struct Packet<double> {
+  using type = double2;
+  static constexpr int amps = 1;
+};
+
+template <typename P>
+__device__ __forceinline__ double packet_re(const P &p, int k);
+template <>
+__device__ __forceinline__ double packet_re<float4>(const float4 &p, int k) { return k ? p.z : p.x; }
+template <>
+__device__ __forceinline__ double packet_re<double2>(const double2 &p, int) { return p.x; }
+template <typename P>
+__device__ __forceinline__ double packet_im(const P &p, int k);
+template <>
+__device__ __forceinline__ double packet_im<float4>(const float4 &p, int k) { return k ? p.w : p.y; }
+template <>
+__device__ __forceinline__ double packet_im<double2>(const double2 &p, int) { return p.y; }
+__device__ __forceinline__ void packet_set(float4 &p, int k, double re, double im) {
+  if (k) {
+    p.z = (float)re;
+    p.w = (float)im;
+  } else {
+    p.x = (float)re;
+    p.y = (float)im;
+  }
+}
+__device__ __forceinline__ void packet_set(double2 &p, int, double re, double im) {
+  p.x = re;
+  p.y = im;
+}
+
 // w <- w - ca*pa - cb*pb ; partial ||w||^2
 template <typename T>
-__global__ __launch_bounds__(kThreads) void lanczos_update_kernel(
-    uint64_t D, typename Vec2<T>::type *w, const typename Vec2<T>::type *pa, double car,
-    double cai, const typename Vec2<T>::type *pb, double cbr, double cbi, double *partials) {
-  using T2 = typename Vec2<T>::type;
+__global__ __launch_bounds__(kThreads) void lanczos_update_kernel(uint64_t npk, void *wv,
+                                                                  const void *pav, double car,
+                                                                  double cai, const void *pbv,
+                                                                  double cbr, double cbi,
+                                                                  double *partials) {
+  using P = typename Packet<T>::type;
+  P *w = reinterpret_cast<P *>(wv);
+  const P *pa = reinterpret_cast<const P *>(pav);
+  const P *pb = reinterpret_cast<const P *>(pbv);
   double nrm[1] = {0.0};
   __shared__ double red[kWaves];
   const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-  for (uint64_t x = (uint64_t)blockIdx.x * kThreads + threadIdx.x; x < D; x += stride) {
-    T2 z = w[x];
-    double zr = z.x, zi = z.y;
-    if (pa) {
-      const T2 p = pa[x];
-      zr -= car * p.x - cai * p.y;
-      zi -= car * p.y + cai * p.x;
+  for (uint64_t x = (uint64_t)blockIdx.x * kThreads + threadIdx.x; x < npk; x += stride) {
+    P z = w[x];
+    P qa, qb;
+    if (pa) qa = pa[x];
+    if (pb) qb = pb[x];
+#pragma unroll
+    for (int k = 0; k < Packet<T>::amps; ++k) {
+      double zr = packet_re(z, k), zi = packet_im(z, k);
+      if (pa) {
+        const double ar = packet_re(qa, k), ai = packet_im(qa, k);
+        zr -= car * ar - cai * ai;
+        zi -= car * ai + cai * ar;
+      }
+      if (pb) {
+        const double br = packet_re(qb, k), bi = packet_im(qb, k);
+        zr -= cbr * br - cbi * bi;
+        zi -= cbr * bi + cbi * br;
+      }
+      packet_set(z, k, zr, zi);
+      const double qr = packet_re(z, k), qi = packet_im(z, k);
+      nrm[0] += qr * qr + qi * qi;
     }
-    if (pb) {
-      const T2 p = pb[x];
-      zr -= cbr * p.x - cbi * p.y;
-      zi -= cbr * p.y + cbi * p.x;
-    }
-    z.x = static_cast<T>(zr);
-    z.y = static_cast<T>(zi);
     w[x] = z;
-    const double qr = z.x, qi = z.y;
-    nrm[0] += qr * qr + qi * qi;
   }
   block_reduce_store<1>(nrm, red, partials + (uint64_t)blockIdx.x * kNumDots);
 }
 
 // ||a||^2 and <a|b>
 template <typename T>
-__global__ __launch_bounds__(kThreads) void dot_kernel(uint64_t D, const typename Vec2<T>::type *a,
-                                                       const typename Vec2<T>::type *b,
+__global__ __launch_bounds__(kThreads) void dot_kernel(uint64_t npk, const void *av, const void *bv,
                                                        double *partials) {
-  using T2 = typename Vec2<T>::type;
+  using P = typename Packet<T>::type;
+  const P *a = reinterpret_cast<const P *>(av);
+  const P *b = reinterpret_cast<const P *>(bv);
   double acc[3] = {0.0, 0.0, 0.0};
   __shared__ double red[kWaves * 3];
   const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-  for (uint64_t x = (uint64_t)blockIdx.x * kThreads + threadIdx.x; x < D; x += stride) {
-    const T2 p = a[x];
-    const double ar = p.x, ai = p.y;
-    acc[0] += ar * ar + ai * ai;
-    if (b) {
-      const T2 q = b[x];
-      const double br = q.x, bi = q.y;
-      acc[1] += ar * br + ai * bi;
-      acc[2] += ar * bi - ai * br;
+  for (uint64_t x = (uint64_t)blockIdx.x * kThreads + threadIdx.x; x < npk; x += stride) {
+    const P p = a[x];
+    P q;
+    if (b) q = b[x];
+#pragma unroll
+    for (int k = 0; k < Packet<T>::amps; ++k) {
+      const double ar = packet_re(p, k), ai = packet_im(p, k);
+      acc[0] += ar * ar + ai * ai;
+      if (b) {
+        const double br = packet_re(q, k), bi = packet_im(q, k);
+        acc[1] += ar * br + ai * bi;
+        acc[2] += ar * bi - ai * br;
+      }
     }
   }
   block_reduce_store<3>(acc, red, partials + (uint64_t)blockIdx.x * kNumDots);
 }
 
 template <typename T>
-__global__ __launch_bounds__(kThreads) void scale_kernel(uint64_t D, typename Vec2<T>::type *v,
-                                                         T s) {
+__global__ __launch_bounds__(kThreads) void scale_kernel(uint64_t npk, void *vv, T s) {
+  using P = typename Packet<T>::type;
+  P *v = reinterpret_cast<P *>(vv);
   const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-  for (uint64_t x = (uint64_t)blockIdx.x * kThreads + threadIdx.x; x < D; x += stride) {
-    auto z = v[x];
-    z.x *= s;
-    z.y *= s;
+  for (uint64_t x = (uint64_t)blockIdx.x * kThreads + threadIdx.x; x < npk; x += stride) {
+    P z = v[x];
+#pragma unroll
+    for (int k = 0; k < Packet<T>::amps; ++k)
+      packet_set(z, k, packet_re(z, k) * s, packet_im(z, k) * s);
     v[x] = z;
   }
 }
@@ -404,26 +762,33 @@ struct LincombArgs {
 };
 
 template <typename T>
-__global__ __launch_bounds__(kThreads) void lincomb_kernel(uint64_t D, const LincombArgs args,
-                                                           typename Vec2<T>::type *out) {
-  using T2 = typename Vec2<T>::type;
+__global__ __launch_bounds__(kThreads) void lincomb_kernel(uint64_t npk, const LincombArgs args,
+                                                           void *outv) {
+  using P = typename Packet<T>::type;
+  constexpr int A = Packet<T>::amps;
+  P *out = reinterpret_cast<P *>(outv);
   const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-  for (uint64_t x = (uint64_t)blockIdx.x * kThreads + threadIdx.x; x < D; x += stride) {
-    double zr = 0.0, zi = 0.0;
-    if (args.accumulate) {
-      const T2 o = out[x];
-      zr = o.x;
-      zi = o.y;
+  for (uint64_t x = (uint64_t)blockIdx.x * kThreads + threadIdx.x; x < npk; x += stride) {
+    double zr[A], zi[A];
+    P o;
+    if (args.accumulate) o = out[x];
+#pragma unroll
+    for (int k = 0; k < A; ++k) {
+      zr[k] = args.accumulate ? packet_re(o, k) : 0.0;
+      zi[k] = args.accumulate ? packet_im(o, k) : 0.0;
     }
-    for (int k = 0; k < args.count; ++k) {
-      const T2 p = reinterpret_cast<const T2 *>(args.vec[k])[x];
-      const double pr = p.x, pi = p.y, cr = args.cr[k], ci = args.ci[k];
-      zr += cr * pr - ci * pi;
-      zi += cr * pi + ci * pr;
+    for (int j = 0; j < args.count; ++j) {
+      const P p = reinterpret_cast<const P *>(args.vec[j])[x];
+      const double cr = args.cr[j], ci = args.ci[j];
+#pragma unroll
+      for (int k = 0; k < A; ++k) {
+        const double pr = packet_re(p, k), pi = packet_im(p, k);
+        zr[k] += cr * pr - ci * pi;
+        zi[k] += cr * pi + ci * pr;
+      }
     }
-    T2 o;
-    o.x = static_cast<T>(zr);
-    o.y = static_cast<T>(zi);
+#pragma unroll
+    for (int k = 0; k < A; ++k) packet_set(o, k, zr[k], zi[k]);
     out[x] = o;
   }
 }
@@ -579,8 +944,18 @@ int make_plan(int n, int tl, int cmin, Pass *out) {
   return np;
 }
 
-void default_tiles(int dtype, int &tl, int &cmin) {
-  if (tl <= 0) tl = dtype == EMU_SV_C64 ? kDefaultTileLogC64 : kDefaultTileLogC128;
+int kernel_kind(int flags) {
+  const int k = (flags >> 8) & 0xF;
+  return k == 0 ? kKernelWave : k;
+}
+
+void default_tiles(int dtype, int kind, int &tl, int &cmin) {
+  if (tl <= 0) {
+    if (kind == kKernelWave)
+      tl = dtype == EMU_SV_C64 ? kDefaultWaveLogC64 : kDefaultWaveLogC128;
+    else
+      tl = dtype == EMU_SV_C64 ? kDefaultTileLogC64 : kDefaultTileLogC128;
+  }
   if (cmin <= 0) cmin = dtype == EMU_SV_C64 ? kDefaultContigC64 : kDefaultContigC128;
 }
 
@@ -610,20 +985,66 @@ int launch_pass(const PassArgs &a, hipStream_t st) {
   return hip_check(hipGetLastError(), "hpsi_pass_kernel launch");
 }
 
+double *partials2_ptr(void *ws, int n);
+double *partials_ptr(void *ws, int n);
+int finalize(const double *rows, int nrow, int K, void *ws, int n, double *out, hipStream_t st);
+const double *dtab_ptr(void *ws, int n);
+
+template <typename T, int KREG, bool CPLX, bool DOTS>
+int launch_wave(const PassArgs &a, hipStream_t st) {
+  const uint64_t nt = 1ull << (a.n - (6 + KREG));
+  const uint64_t nblk = nt >= (uint64_t)kWaves ? nt / kWaves : 1;
+  hipLaunchKernelGGL((hpsi_wave_kernel<T, KREG, CPLX, DOTS>), dim3((unsigned)nblk), dim3(kThreads),
+                     0, st, a);
+  return hip_check(hipGetLastError(), "hpsi_wave_kernel launch");
+}
+
+template <typename T, int KREG>
+int launch_wave_any(const PassArgs &a, bool cplx, bool dots, hipStream_t st) {
+  if (cplx)
+    return dots ? launch_wave<T, KREG, true, true>(a, st) : launch_wave<T, KREG, true, false>(a, st);
+  return dots ? launch_wave<T, KREG, false, true>(a, st) : launch_wave<T, KREG, false, false>(a, st);
+}
+
+struct OrthoIn {
+  const void *pa, *pb;
+  double ca_re, ca_im, cb_re, cb_im;
+  double *norm2_out;
+};
+
 template <typename T>
 int run_hpsi(int n, const double *ham, double sr, double si, const void *v, double vsc,
-             const void *prev, double psc, void *w, double *dots, void *ws, int tl, int cmin,
-             bool cplx, hipStream_t st) {
+             const void *prev, double psc, void *w, double *dots, void *ws, int kind, int tl,
+             int cmin, bool cplx, hipStream_t st, const OrthoIn *ortho = nullptr) {
+  if (kind == kKernelWave && n < tl) kind = kKernelLds;  // one small tile: LDS kernel
   Pass plan[EMU_SV_MAX_QUBITS];
   const int np = make_plan(n, tl, cmin, plan);
+  {
+    const int t1 = plan[0].c + plan[0].m;  // pass 1 is the contiguous tile [0, t1)
+    const uint64_t ntiles = 1ull << (n - t1);
+    hipLaunchKernelGGL(diag_prep_kernel, dim3((unsigned)((ntiles + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, st, n, t1, ham,
+                       const_cast<double *>(dtab_ptr(ws, n)));
+    if (int rc = hip_check(hipGetLastError(), "diag_prep launch")) return rc;
+  }
   for (int p = 0; p < np; ++p) {
     PassArgs a;
     a.v = v;
     a.prev = prev;
     a.out = w;
     a.ham = ham;
-    a.partials = reinterpret_cast<double *>(ws);
-    a.v_scale = vsc;
+    a.partials = partials_ptr(ws, n);
+    a.dtab = dtab_ptr(ws, n);
+    a.ortho = (ortho != nullptr && p == 0) ? 1 : 0;
+    a.zout = const_cast<void *>(v);
+    a.pa = ortho ? ortho->pa : nullptr;
+    a.pb = ortho ? ortho->pb : nullptr;
+    a.ca_re = ortho ? ortho->ca_re : 0.0;
+    a.ca_im = ortho ? ortho->ca_im : 0.0;
+    a.cb_re = ortho ? ortho->cb_re : 0.0;
+    a.cb_im = ortho ? ortho->cb_im : 0.0;
+    a.partials2 = partials2_ptr(ws, n);
+    a.v_scale = (ortho != nullptr && p > 0) ? 1.0 : vsc;
     a.prev_scale = psc;
     a.sig_re = sr;
     a.sig_im = si;
@@ -638,22 +1059,85 @@ int run_hpsi(int n, const double *ham, double sr, double si, const void *v, doub
     const bool last = p == np - 1;
     const bool fuse = last && dots != nullptr;
     int rc;
-    if (cplx)
+    if (kind == kKernelWave) {
+      switch (tl - 6) {
+        case 3: rc = launch_wave_any<T, 3>(a, cplx, fuse, st); break;
+        case 4: rc = launch_wave_any<T, 4>(a, cplx, fuse, st); break;
+        case 5: rc = launch_wave_any<T, 5>(a, cplx, fuse, st); break;
+        default: return fail(EMU_SV_EINVAL, "hpsi: wave tiles need tile_log in [9, 11]");
+      }
+    } else if (cplx) {
       rc = fuse ? launch_pass<T, true, true>(a, st) : launch_pass<T, true, false>(a, st);
-    else
+    } else {
       rc = fuse ? launch_pass<T, false, true>(a, st) : launch_pass<T, false, false>(a, st);
+    }
     if (rc) return rc;
+    const int t = a.c + a.m;
+    const int nrow = (int)(1ull << (n - t));
+    if (a.ortho) {
+      rc = finalize(a.partials2, nrow, 1, ws, n, ortho->norm2_out, st);
+      if (rc) return rc;
+    }
     if (fuse) {
-      const int t = a.c + a.m;
-      const int nblk = (int)(1ull << (n - t));
-      hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, st,
-                         reinterpret_cast<const double *>(ws), nblk, kNumDots, 7, dots);
-      rc = hip_check(hipGetLastError(), "finalize launch");
+      rc = finalize(a.partials, nrow, 7, ws, n, dots, st);
       if (rc) return rc;
     }
   }
   return EMU_SV_OK;
 }
+
+// Workspace layout (bytes): [partial rows | pass-1 diagonal table |
+// finalize stage | ticket counter].  The caller zero-fills it once; the
+// ticket is re-armed by every reduction.
+constexpr int kCorrBlocks = 1024;
+struct WsLayout {
+  size_t partials, partials2, dtab, stage, ticket, total;
+};
+
+WsLayout ws_layout(int n) {
+  // The ticket and the finalize stage sit at fixed offsets, so a workspace
+  // sized for a larger n can serve any smaller n without stale counters.
+  const uint64_t tiles = n > kThreadBits ? (1ull << (n - kThreadBits)) : 1ull;
+  const uint64_t rows = tiles > (uint64_t)kStreamBlocks ? tiles : (uint64_t)kStreamBlocks;
+  uint64_t part = rows * kNumDots * sizeof(double);
+  const uint64_t corr = (uint64_t)kCorrBlocks * EMU_SV_MAX_QUBITS * EMU_SV_MAX_QUBITS * sizeof(double);
+  if (corr > part) part = corr;
+  WsLayout l;
+  l.ticket = 0;
+  l.stage = 256;
+  l.partials = l.stage + kFinalizeBlocks * kNumDots * sizeof(double);
+  l.partials2 = l.partials + part;
+  l.dtab = l.partials2 + tiles * kNumDots * sizeof(double);
+  l.total = l.dtab + tiles * kDiagRow * sizeof(double);
+  return l;
+}
+
+double *partials_ptr(void *ws, int n) {
+  return reinterpret_cast<double *>(reinterpret_cast<char *>(ws) + ws_layout(n).partials);
+}
+
+double *partials2_ptr(void *ws, int n) {
+  return reinterpret_cast<double *>(reinterpret_cast<char *>(ws) + ws_layout(n).partials2);
+}
+
+const double *dtab_ptr(void *ws, int n) {
+  return reinterpret_cast<const double *>(reinterpret_cast<char *>(ws) + ws_layout(n).dtab);
+}
+
+int finalize(const double *rows, int nrow, int K, void *ws, int n, double *out, hipStream_t st) {
+  const WsLayout l = ws_layout(n);
+  char *base = reinterpret_cast<char *>(ws);
+  int nb = (nrow + 31) / 32;
+  if (nb > kFinalizeBlocks) nb = kFinalizeBlocks;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(finalize2_kernel, dim3(nb), dim3(kThreads), 0, st, rows, nrow, K,
+                     reinterpret_cast<double *>(base + l.stage),
+                     reinterpret_cast<unsigned *>(base + l.ticket), out);
+  return hip_check(hipGetLastError(), "finalize launch");
+}
+
+// 16-byte packets in a 2^n vector (c64: two amplitudes per packet; n >= 1).
+uint64_t packets(int n, int dtype) { return dtype == EMU_SV_C64 ? (1ull << (n - 1)) : (1ull << n); }
 
 int stream_blocks(uint64_t D) {
   uint64_t b = (D + kThreads - 1) / kThreads;
@@ -709,10 +1193,10 @@ int emu_sv_pack_hamiltonian(int n, const double *omega, const double *delta, con
   return EMU_SV_OK;
 }
 
-int emu_sv_plan_passes(int n, int dtype, int tile_log, int min_contig) {
+int emu_sv_plan_passes(int n, int dtype, int tile_log, int min_contig, int flags) {
   if (int rc = check_n(n)) return rc;
   if (int rc = check_dtype(dtype)) return rc;
-  default_tiles(dtype, tile_log, min_contig);
+  default_tiles(dtype, kernel_kind(flags), tile_log, min_contig);
   Pass plan[EMU_SV_MAX_QUBITS];
   return make_plan(n, tile_log, min_contig, plan);
 }
@@ -720,11 +1204,7 @@ int emu_sv_plan_passes(int n, int dtype, int tile_log, int min_contig) {
 size_t emu_sv_workspace_bytes(int n, int dtype) {
   (void)dtype;
   if (n < 1 || n > EMU_SV_MAX_QUBITS) return 0;
-  const uint64_t tiles = n > kThreadBits ? (1ull << (n - kThreadBits)) : 1ull;
-  uint64_t rows = tiles > (uint64_t)kStreamBlocks ? tiles : (uint64_t)kStreamBlocks;
-  uint64_t bytes = rows * kNumDots * sizeof(double);
-  const uint64_t corr = (uint64_t)kStreamBlocks * EMU_SV_MAX_QUBITS * EMU_SV_MAX_QUBITS * sizeof(double);
-  return (size_t)(bytes > corr ? bytes : corr);
+  return ws_layout(n).total;
 }
 
 int emu_sv_hpsi(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
@@ -736,7 +1216,12 @@ int emu_sv_hpsi(int n, int dtype, const double *ham, double sigma_re, double sig
   if (!ham || !v || !w) return fail(EMU_SV_EINVAL, "hpsi: null pointer");
   if (v == w) return fail(EMU_SV_EINVAL, "hpsi: input and output must not alias");
   if (dots_out && !workspace) return fail(EMU_SV_EINVAL, "hpsi: reductions need a workspace");
-  default_tiles(dtype, tile_log, min_contig);
+  const int kind = kernel_kind(flags);
+  if (kind != kKernelLds && kind != kKernelWave)
+    return fail(EMU_SV_EINVAL, "hpsi: unknown kernel kind in flags");
+  default_tiles(dtype, kind, tile_log, min_contig);
+  if (kind == kKernelWave && (tile_log < 9 || tile_log > 11))
+    return fail(EMU_SV_EINVAL, "hpsi: wave tiles need tile_log in [9, 11]");
   if (tile_log < kThreadBits || tile_log > kMaxTileLog)
     return fail(EMU_SV_EINVAL, "hpsi: tile_log must be in [8, 14]");
   if (dtype == EMU_SV_C128 && tile_log > 13)
@@ -746,9 +1231,45 @@ int emu_sv_hpsi(int n, int dtype, const double *ham, double sigma_re, double sig
   const bool cplx = (flags & EMU_SV_FLAG_COMPLEX) != 0;
   if (dtype == EMU_SV_C128)
     return run_hpsi<double>(n, ham, sigma_re, sigma_im, v, v_scale, prev, prev_scale, w, dots_out,
-                            workspace, tile_log, min_contig, cplx, stream);
+                            workspace, kind, tile_log, min_contig, cplx, stream);
   return run_hpsi<float>(n, ham, sigma_re, sigma_im, v, v_scale, prev, prev_scale, w, dots_out,
-                         workspace, tile_log, min_contig, cplx, stream);
+                         workspace, kind, tile_log, min_contig, cplx, stream);
+}
+
+int emu_sv_lanczos_matvec(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
+                          void *w, double w_scale, const void *pa, double pa_scale, double a_re,
+                          double a_im, const void *pb, double pb_scale, double b_re, double b_im,
+                          void *r, double *norm2_out, double *dots_out, void *workspace,
+                          int tile_log, int min_contig, int flags, hipStream_t stream) {
+  if (int rc = check_n(n)) return rc;
+  if (int rc = check_dtype(dtype)) return rc;
+  if (!ham || !w || !r || !norm2_out || !dots_out || !workspace)
+    return fail(EMU_SV_EINVAL, "lanczos_matvec: null pointer");
+  if (w == r || w == pa || w == pb || r == pa || r == pb)
+    return fail(EMU_SV_EINVAL, "lanczos_matvec: w, r, pa and pb must be distinct buffers");
+  const int kind = kernel_kind(flags);
+  if (kind != kKernelLds && kind != kKernelWave)
+    return fail(EMU_SV_EINVAL, "lanczos_matvec: unknown kernel kind in flags");
+  default_tiles(dtype, kind, tile_log, min_contig);
+  if (kind == kKernelWave && (tile_log < 9 || tile_log > 11))
+    return fail(EMU_SV_EINVAL, "lanczos_matvec: wave tiles need tile_log in [9, 11]");
+  if (tile_log < kThreadBits || tile_log > kMaxTileLog ||
+      (dtype == EMU_SV_C128 && tile_log > 13) || min_contig < 0 || min_contig >= tile_log)
+    return fail(EMU_SV_EINVAL, "lanczos_matvec: bad tile_log / min_contig");
+  OrthoIn o;
+  o.pa = pa;
+  o.pb = pb;
+  o.ca_re = a_re * pa_scale;
+  o.ca_im = a_im * pa_scale;
+  o.cb_re = b_re * pb_scale;
+  o.cb_im = b_im * pb_scale;
+  o.norm2_out = norm2_out;
+  const bool cplx = (flags & EMU_SV_FLAG_COMPLEX) != 0;
+  if (dtype == EMU_SV_C128)
+    return run_hpsi<double>(n, ham, sigma_re, sigma_im, w, w_scale, pb, pb_scale, r, dots_out,
+                            workspace, kind, tile_log, min_contig, cplx, stream, &o);
+  return run_hpsi<float>(n, ham, sigma_re, sigma_im, w, w_scale, pb, pb_scale, r, dots_out,
+                         workspace, kind, tile_log, min_contig, cplx, stream, &o);
 }
 
 int emu_sv_lanczos_update(int n, int dtype, void *w, const void *pa, double pa_scale, double a_re,
@@ -757,21 +1278,19 @@ int emu_sv_lanczos_update(int n, int dtype, void *w, const void *pa, double pa_s
   if (int rc = check_n(n)) return rc;
   if (int rc = check_dtype(dtype)) return rc;
   if (!w || !norm2_out || !workspace) return fail(EMU_SV_EINVAL, "lanczos_update: null pointer");
-  const uint64_t D = 1ull << n;
-  const int nb = stream_blocks(D);
-  double *part = reinterpret_cast<double *>(workspace);
+  const uint64_t npk = packets(n, dtype);
+  const int nb = stream_blocks(npk);
+  double *part = partials_ptr(workspace, n);
   if (dtype == EMU_SV_C128)
-    hipLaunchKernelGGL(lanczos_update_kernel<double>, dim3(nb), dim3(kThreads), 0, stream, D,
-                       (double2 *)w, (const double2 *)pa, a_re * pa_scale, a_im * pa_scale,
-                       (const double2 *)pb, b_re * pb_scale, b_im * pb_scale, part);
+    hipLaunchKernelGGL(lanczos_update_kernel<double>, dim3(nb), dim3(kThreads), 0, stream, npk, w,
+                       pa, a_re * pa_scale, a_im * pa_scale, pb, b_re * pb_scale,
+                       b_im * pb_scale, part);
   else
-    hipLaunchKernelGGL(lanczos_update_kernel<float>, dim3(nb), dim3(kThreads), 0, stream, D,
-                       (float2 *)w, (const float2 *)pa, a_re * pa_scale, a_im * pa_scale,
-                       (const float2 *)pb, b_re * pb_scale, b_im * pb_scale, part);
+    hipLaunchKernelGGL(lanczos_update_kernel<float>, dim3(nb), dim3(kThreads), 0, stream, npk, w,
+                       pa, a_re * pa_scale, a_im * pa_scale, pb, b_re * pb_scale,
+                       b_im * pb_scale, part);
   if (int rc = hip_check(hipGetLastError(), "lanczos_update launch")) return rc;
-  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, stream, part, nb, kNumDots, 1,
-                     norm2_out);
-  return hip_check(hipGetLastError(), "finalize launch");
+  return finalize(part, nb, 1, workspace, n, norm2_out, stream);
 }
 
 int emu_sv_lincomb(int n, int dtype, int count, const void *const *vecs, const double *coef,
@@ -786,8 +1305,8 @@ int emu_sv_lincomb(int n, int dtype, int count, const void *const *vecs, const d
     if (k > 0 && vecs[k] == out)
       return fail(EMU_SV_EINVAL, "lincomb: only the first vector may alias the output");
   }
-  const uint64_t D = 1ull << n;
-  const int nb = stream_blocks(D);
+  const uint64_t npk = packets(n, dtype);
+  const int nb = stream_blocks(npk);
   for (int k0 = 0; k0 < count; k0 += kLincombBatch) {
     LincombArgs args;
     memset(&args, 0, sizeof(args));
@@ -799,11 +1318,11 @@ int emu_sv_lincomb(int n, int dtype, int count, const void *const *vecs, const d
       args.ci[k] = coef[2 * (k0 + k) + 1];
     }
     if (dtype == EMU_SV_C128)
-      hipLaunchKernelGGL(lincomb_kernel<double>, dim3(nb), dim3(kThreads), 0, stream, D, args,
-                         (double2 *)out);
+      hipLaunchKernelGGL(lincomb_kernel<double>, dim3(nb), dim3(kThreads), 0, stream, npk, args,
+                         out);
     else
-      hipLaunchKernelGGL(lincomb_kernel<float>, dim3(nb), dim3(kThreads), 0, stream, D, args,
-                         (float2 *)out);
+      hipLaunchKernelGGL(lincomb_kernel<float>, dim3(nb), dim3(kThreads), 0, stream, npk, args,
+                         out);
     if (int rc = hip_check(hipGetLastError(), "lincomb launch")) return rc;
   }
   return EMU_SV_OK;
@@ -814,32 +1333,27 @@ int emu_sv_dot(int n, int dtype, const void *a, const void *b, double *dots_out,
   if (int rc = check_n(n)) return rc;
   if (int rc = check_dtype(dtype)) return rc;
   if (!a || !dots_out || !workspace) return fail(EMU_SV_EINVAL, "dot: null pointer");
-  const uint64_t D = 1ull << n;
-  const int nb = stream_blocks(D);
-  double *part = reinterpret_cast<double *>(workspace);
+  const uint64_t npk = packets(n, dtype);
+  const int nb = stream_blocks(npk);
+  double *part = partials_ptr(workspace, n);
   if (dtype == EMU_SV_C128)
-    hipLaunchKernelGGL(dot_kernel<double>, dim3(nb), dim3(kThreads), 0, stream, D,
-                       (const double2 *)a, (const double2 *)b, part);
+    hipLaunchKernelGGL(dot_kernel<double>, dim3(nb), dim3(kThreads), 0, stream, npk, a, b, part);
   else
-    hipLaunchKernelGGL(dot_kernel<float>, dim3(nb), dim3(kThreads), 0, stream, D,
-                       (const float2 *)a, (const float2 *)b, part);
+    hipLaunchKernelGGL(dot_kernel<float>, dim3(nb), dim3(kThreads), 0, stream, npk, a, b, part);
   if (int rc = hip_check(hipGetLastError(), "dot launch")) return rc;
-  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, stream, part, nb, kNumDots, 3,
-                     dots_out);
-  return hip_check(hipGetLastError(), "finalize launch");
+  return finalize(part, nb, 3, workspace, n, dots_out, stream);
 }
 
 int emu_sv_scale(int n, int dtype, void *v, double s, hipStream_t stream) {
   if (int rc = check_n(n)) return rc;
   if (int rc = check_dtype(dtype)) return rc;
   if (!v) return fail(EMU_SV_EINVAL, "scale: null pointer");
-  const uint64_t D = 1ull << n;
-  const int nb = stream_blocks(D);
+  const uint64_t npk = packets(n, dtype);
+  const int nb = stream_blocks(npk);
   if (dtype == EMU_SV_C128)
-    hipLaunchKernelGGL(scale_kernel<double>, dim3(nb), dim3(kThreads), 0, stream, D, (double2 *)v, s);
+    hipLaunchKernelGGL(scale_kernel<double>, dim3(nb), dim3(kThreads), 0, stream, npk, v, s);
   else
-    hipLaunchKernelGGL(scale_kernel<float>, dim3(nb), dim3(kThreads), 0, stream, D, (float2 *)v,
-                       (float)s);
+    hipLaunchKernelGGL(scale_kernel<float>, dim3(nb), dim3(kThreads), 0, stream, npk, v, (float)s);
   return hip_check(hipGetLastError(), "scale launch");
 }
 
@@ -873,8 +1387,8 @@ int emu_sv_correlation(int n, int dtype, const void *psi, double *corr_out, void
   if (!psi || !corr_out || !workspace) return fail(EMU_SV_EINVAL, "correlation: null pointer");
   const int K = n < kMaxTileLog ? n : kMaxTileLog;
   const uint64_t nchunks = 1ull << (n - K);
-  const int nb = (int)(nchunks < (uint64_t)kStreamBlocks ? nchunks : (uint64_t)kStreamBlocks);
-  double *part = reinterpret_cast<double *>(workspace);
+  const int nb = (int)(nchunks < (uint64_t)kCorrBlocks ? nchunks : (uint64_t)kCorrBlocks);
+  double *part = partials_ptr(workspace, n);
   if (dtype == EMU_SV_C128)
     hipLaunchKernelGGL(correlation_kernel<double>, dim3(nb), dim3(kThreads), 0, stream, n,
                        (const double2 *)psi, part);

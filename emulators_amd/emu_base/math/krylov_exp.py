@@ -11,14 +11,17 @@ Two implementations sit behind it:
 
 * ``op`` a ``ScaledHamiltonian`` (op(x) = sigma H x for a
   ``RydbergHamiltonian``) with a device vector and ``is_hermitian``: the
-  fused engine.  Each Lanczos iteration is one H.psi launch sequence (LDS
-  tiled passes, the diagonal generated on the fly) whose last pass also
-  reduces ||w||^2, <v_j|w>, <v_{j-1}|w> and <v_j|v_{j-1}>, then one update
-  launch  w <- w - T[j-1,j] v_{j-1} - T[j,j] v_j  that reduces ||w||^2.
-  Basis vectors are stored unnormalised with a host-side scale (the
-  normalisation is folded into the next launch), the tridiagonal T, its
-  exponential and the error estimate are complex128 on the host, and the
-  result is one linear-combination launch.  The arithmetic is the
+  fused engine.  Each Lanczos iteration is one H.psi launch sequence
+  (tiled passes, the diagonal generated on the fly): its first pass
+  orthogonalises the previous iteration's vector,
+  z = w - T[j-1,j] v_{j-1} - T[j,j] v_j, reducing ||z||^2, and its last pass
+  reduces ||Hz||^2, <z|Hz>, <v_j|Hz> and <z|v_j>; one read-back per
+  iteration.  Basis vectors are stored unnormalised with a host-side scale
+  (the normalisation is folded into the next launch).  T is complex128 on
+  the host; its exponential and the Expokit error terms come from the
+  symmetric tridiagonal form (emu_sv_lanczos_exp: T = W^-1 sigma J W with J
+  real symmetric, O(m^2) per iteration instead of a dense (j+3)^2 expm), and
+  the result is one linear-combination launch.  The arithmetic is the
   reference's (two-vector modified Gram-Schmidt, T[j-1,j] taken first and
   T[j,j] = <v_j | w - T[j-1,j] v_{j-1}>).  Vectors may be complex64: the
   Lanczos scalars, reductions and T stay FP64.
@@ -37,6 +40,7 @@ import numpy as np
 import torch
 
 from emulators_amd import _native as native
+from emulators_amd import autotune
 
 DEFAULT_MAX_KRYLOV_DIM: int = 100
 
@@ -90,11 +94,22 @@ def _error_estimate(expd: np.ndarray, j: int, n: float) -> float:
 
 def _fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
                    norm_tolerance: float, max_krylov_dim: int) -> KrylovExpResult:
+    """Lanczos of krylov_exp.py:23-106 with one fused launch sequence and one
+    read-back per iteration.
+
+    Call j computes, in its first H.psi pass, the orthogonalised vector
+    z_j = w'_j - T[j-1,j] v_{j-1} - T[j,j] v_j (stored as the next basis
+    vector, unnormalised) and then r = sigma H z_j, with ||z_j||^2 and the
+    dots of r that give T[j,j+1] and T[j+1,j+1].  The convergence test of
+    iteration j therefore lands after the matvec of iteration j+1: one matvec
+    per Krylov step is computed and not used."""
     ham = op.hamiltonian
     n = native.nqubits_of(v)
     if n != ham.nqubits:
         raise ValueError(f"state has {n} qubits, the Hamiltonian {ham.nqubits}")
     params = ham.device_params(v.device)
+    sched = autotune.best_schedule(n, v.dtype, v.device, params, ham.complex)
+    kw = sched.kwargs()
     KrylovStats.calls += 1
 
     initial_norm = math.sqrt(native.dot(n, v)[0].item())
@@ -108,37 +123,52 @@ def _fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
         out = torch.empty_like(v)
         return native.lincomb(n, basis[:k], c, out)
 
-    expd = None
+    # w'_0 = op(v_0)
+    wbuf = torch.empty_like(v)
+    d = native.hpsi(n, params, op.sigma, v, scales[0], wbuf, dots=True, complex_coef=ham.complex,
+                    **kw).cpu().numpy()
+    KrylovStats.matvecs += 1
+    w_scale = 1.0
+    ww, vw, pw, vp = d[0], complex(d[1], d[2]), 0j, 0j
+    coeffs = None
     for j in range(max_krylov_dim):
-        w = torch.empty_like(v)
-        prev = basis[j - 1] if j > 0 else None
-        prev_scale = scales[j - 1] if j > 0 else 0.0
-        dots = native.hpsi(n, params, op.sigma, basis[j], scales[j], w, prev=prev,
-                           prev_scale=prev_scale, dots=True, complex_coef=ham.complex)
-        KrylovStats.matvecs += 1
-        d = dots.cpu().numpy()
-        nrm = math.sqrt(max(d[0], 0.0))  # ||op(v_j)|| before orthogonalisation
+        nrm = math.sqrt(max(ww, 0.0))  # ||op(v_j)|| before orthogonalisation (:52)
         if j > 0:
-            a = complex(d[3], d[4])
+            a = pw
             T[j - 1, j] = a
-            b = complex(d[1], d[2]) - a * complex(d[5], d[6])
+            b = vw - a * vp
         else:
-            a = 0j
-            b = complex(d[1], d[2])
+            a, b = 0j, vw
         T[j, j] = b
-        n2sq = native.lanczos_update(n, w, prev, prev_scale, a, basis[j], scales[j], b).item()
-        n2 = math.sqrt(max(n2sq, 0.0))
+        r = torch.empty_like(v)
+        o = native.lanczos_matvec(n, params, op.sigma, wbuf, w_scale,
+                                  basis[j - 1] if j > 0 else None,
+                                  scales[j - 1] if j > 0 else 0.0, a, basis[j], scales[j], b, r,
+                                  complex_coef=ham.complex, **kw).cpu().numpy()
+        KrylovStats.matvecs += 1
+        n2 = math.sqrt(max(o[0], 0.0))
         T[j + 1, j] = n2
-        if n2 < norm_tolerance:  # happy breakdown
-            expd = native.expm(T[: j + 1, : j + 1])
-            return KrylovExpResult(combine(expd[:, 0]), True, True, j + 1)
-        basis.append(w)
+        if n2 < norm_tolerance:  # happy breakdown (:63-71)
+            _, _, c = native.lanczos_exp(T, j + 1, op.sigma, 0.0, True)
+            return KrylovExpResult(combine(c[: j + 1]), True, True, j + 1)
+        basis.append(wbuf)  # z_j, normalised by its scale 1/n2
         scales.append(1.0 / n2)
         T[j + 2, j + 1] = 1.0
-        expd = native.expm(T[: j + 3, : j + 3])
-        if _error_estimate(expd, j, nrm) < exp_tolerance:
-            return KrylovExpResult(combine(expd[: len(basis), 0]), True, False, j + 1)
-    return KrylovExpResult(combine(expd[: len(basis), 0]), False, False, max_krylov_dim)
+        # exp of the extended matrix: the two Expokit terms (:77-84)
+        e1, e2, _ = native.lanczos_exp(T, j + 1, op.sigma, n2, False)
+        err2 = e2 * nrm
+        err = e1 if e1 < err2 else (e1 * err2 / (e1 - err2))
+        if err < exp_tolerance:
+            _, _, coeffs = native.lanczos_exp(T, j + 1, op.sigma, n2, True)
+            return KrylovExpResult(combine(coeffs), True, False, j + 1)
+        # dots of w'_{j+1} = r / n2 against v_{j+1} = z / n2 and v_j
+        wbuf, w_scale = r, 1.0 / n2
+        ww = o[1] / (n2 * n2)
+        vw = complex(o[2], o[3]) / (n2 * n2)
+        pw = complex(o[4], o[5]) / n2
+        vp = complex(o[6], o[7]) / n2
+    _, _, coeffs = native.lanczos_exp(T, max_krylov_dim, op.sigma, T[max_krylov_dim, max_krylov_dim - 1].real, True)
+    return KrylovExpResult(combine(coeffs), False, False, max_krylov_dim)
 
 
 def _generic_krylov(op: Callable, v: torch.Tensor, is_hermitian: bool, exp_tolerance: float,

@@ -17,6 +17,7 @@ from __future__ import annotations
 import torch
 
 from emulators_amd import _native as native
+from emulators_amd import autotune
 
 
 def _real_params(x: torch.Tensor | float, what: str) -> torch.Tensor:
@@ -77,8 +78,12 @@ class RydbergHamiltonian:
         if out is None:
             out = torch.empty_like(vec)
         native.hpsi(n, self.device_params(vec.device), complex(sigma), vec, 1.0, out,
-                    complex_coef=self.complex)
+                    complex_coef=self.complex, **self._schedule(vec).kwargs())
         return out
+
+    def _schedule(self, vec: torch.Tensor) -> autotune.Schedule:
+        return autotune.best_schedule(self.nqubits, vec.dtype, vec.device,
+                                      self.device_params(vec.device), self.complex)
 
     def __mul__(self, vec: torch.Tensor) -> torch.Tensor:
         return self.apply(vec)
@@ -89,7 +94,7 @@ class RydbergHamiltonian:
         n = native.nqubits_of(vec)
         out = torch.empty_like(vec)
         d = native.hpsi(n, self.device_params(vec.device), 1.0, vec, 1.0, out, dots=True,
-                        complex_coef=self.complex).cpu()
+                        complex_coef=self.complex, **self._schedule(vec).kwargs()).cpu()
         return float(native.dot(n, vec)[0].item()), float(d[1]), float(d[0])
 
     @property
@@ -114,7 +119,7 @@ class RydbergHamiltonian:
         n = native.nqubits_of(data)
         out = torch.empty_like(data)
         d = native.hpsi(n, self.device_params(data.device), 1.0, data, 1.0, out, dots=True,
-                        complex_coef=self.complex).cpu()
+                        complex_coef=self.complex, **self._schedule(data).kwargs()).cpu()
         en = complex(d[1], d[2])
         assert abs(en.imag) <= 1e-8 * max(1.0, abs(en.real)), "<psi|H|psi> is not real"
         return torch.tensor(en.real, dtype=torch.float64)

@@ -17,6 +17,7 @@ import numpy as np
 import torch
 
 from emulators_amd import _native as native
+from emulators_amd import autotune
 from emulators_amd.emu_base.math.krylov_exp import (
     KrylovStats,
     ScaledHamiltonian,
@@ -33,6 +34,7 @@ def lanczos_basis(op: ScaledHamiltonian, v: torch.Tensor, tolerance: float,
     ham = op.hamiltonian
     n = native.nqubits_of(v)
     params = ham.device_params(v.device)
+    sched = autotune.best_schedule(n, v.dtype, v.device, params, ham.complex)
     v0 = v / v.norm()
     vecs = [v0]
     T = np.zeros((max_krylov_dim + 2, max_krylov_dim + 2), dtype=np.complex128)
@@ -41,7 +43,7 @@ def lanczos_basis(op: ScaledHamiltonian, v: torch.Tensor, tolerance: float,
         w = torch.empty_like(v0)
         prev = vecs[j - 1] if j > 0 else None
         d = native.hpsi(n, params, op.sigma, vecs[j], 1.0, w, prev=prev, prev_scale=1.0,
-                        dots=True, complex_coef=ham.complex).cpu().numpy()
+                        dots=True, complex_coef=ham.complex, **sched.kwargs()).cpu().numpy()
         KrylovStats.matvecs += 1
         nrm = math.sqrt(max(d[0], 0.0))
         a = complex(d[3], d[4]) if j > 0 else 0j

@@ -41,8 +41,12 @@ extern "C" {
 #define EMU_SV_C128 0
 #define EMU_SV_C64 1
 
-/* emu_sv_hpsi flags */
+/* emu_sv_hpsi flags: bit 0 complex coefficients, bits 8-11 kernel kind */
 #define EMU_SV_FLAG_COMPLEX 1 /* some phi != 0: complex flip coefficients */
+#define EMU_SV_KERNEL_DEFAULT (0 << 8) /* library choice (the wave kernel)  */
+#define EMU_SV_KERNEL_LDS (1 << 8)     /* workgroup tiles staged in LDS     */
+#define EMU_SV_KERNEL_WAVE (2 << 8)    /* wavefront tiles in registers,     */
+                                       /* cross-lane flips; tile_log 9..11  */
 
 #define EMU_SV_MAX_QUBITS 32
 /* Packed Hamiltonian parameter block, indexed by BIT position b = N-1-q:
@@ -70,9 +74,9 @@ int emu_sv_pack_hamiltonian(int n, const double *omega, const double *delta,
                             const double *phi, const double *interaction,
                             double *block_out);
 
-/* Number of H.psi passes the planner uses for n qubits and element type
- * (each pass streams the vector once through LDS tiles). */
-int emu_sv_plan_passes(int n, int dtype, int tile_log, int min_contig);
+/* Number of H.psi passes the planner uses for n qubits, element type and
+ * schedule (each pass streams the vector once). */
+int emu_sv_plan_passes(int n, int dtype, int tile_log, int min_contig, int flags);
 
 /* Bytes of the per-workgroup partial-sum workspace needed by the calls
  * below for a vector of 2^n elements. */
@@ -87,16 +91,37 @@ size_t emu_sv_workspace_bytes(int n, int dtype);
  *   dots_out device pointer to 7 doubles receiving, after the call:
  *            [0] ||w||^2, [1,2] <v|w>, [3,4] <prev|w>, [5,6] <v|prev>
  *            (v and prev taken with their scales); NULL = no reductions.
- *   workspace device scratch of emu_sv_workspace_bytes(n, dtype) bytes.
- *   tile_log / min_contig: LDS tile size and minimum contiguous bits of
- *            the high-bit passes; 0 = library default.
- *   flags    EMU_SV_FLAG_COMPLEX when any phi != 0 (emu_sv/hamiltonian.py:61).
+ *   workspace device scratch of emu_sv_workspace_bytes(n, dtype) bytes,
+ *            zero-filled before its first use (it holds a counter that the
+ *            library re-arms after every reduction).
+ *   tile_log / min_contig: log2 of the tile (the amplitudes one workgroup
+ *            (LDS kernel) or one wavefront (wave kernel) owns) and the
+ *            minimum contiguous low bits of the high-bit passes; 0 = default.
+ *   flags    EMU_SV_FLAG_COMPLEX when any phi != 0 (emu_sv/hamiltonian.py:61),
+ *            or'ed with an EMU_SV_KERNEL_* kind.
  */
 int emu_sv_hpsi(int n, int dtype, const double *ham, double sigma_re,
                 double sigma_im, const void *v, double v_scale,
                 const void *prev, double prev_scale, void *w, double *dots_out,
                 void *workspace, int tile_log, int min_contig, int flags,
                 hipStream_t stream);
+
+/* One fused Lanczos iteration step (orthogonalise, then the next matvec):
+ *   z = w_scale*w - a*(pa_scale*pa) - b*(pb_scale*pb)   (stored back into w)
+ *   r = sigma*H*z
+ * norm2_out[0] = ||z||^2; dots_out[0..6] as emu_sv_hpsi with v = z and
+ * prev = pb (scale pb_scale).  The orthogonalisation of iteration j
+ * (emu_base/math/krylov_exp.py:54-61, two-vector MGS with coefficients from
+ * the previous call's dots) runs inside the first H.psi pass of iteration
+ * j+1 (krylov_exp.py:50-52), so each Lanczos iteration is one call and one
+ * host read-back.  w, r, pa, pb must be distinct; pa may be NULL. */
+int emu_sv_lanczos_matvec(int n, int dtype, const double *ham, double sigma_re,
+                          double sigma_im, void *w, double w_scale,
+                          const void *pa, double pa_scale, double a_re,
+                          double a_im, const void *pb, double pb_scale,
+                          double b_re, double b_im, void *r, double *norm2_out,
+                          double *dots_out, void *workspace, int tile_log,
+                          int min_contig, int flags, hipStream_t stream);
 
 /* w <- w - a*(pa_scale*pa) - b*(pb_scale*pb); norm2_out[0] = ||w||^2.
  * The orthogonalisation of krylov_exp.py:54-61 (two-vector MGS, with the
@@ -138,6 +163,22 @@ int emu_sv_correlation(int n, int dtype, const void *psi, double *corr_out,
  * Replaces torch.linalg.matrix_exp on the (j+3)x(j+3) extended Krylov
  * matrix (emu_base/math/krylov_exp.py:65,78). */
 int emu_sv_expm(int m, const double *a, double *e);
+
+/* Host: the exponential of a Hermitian Lanczos matrix from its symmetric
+ * tridiagonal form.  tdiag: m complex T[k,k] (interleaved), tsub: m-1 real
+ * T[k+1,k]; sigma: the operator scale (op = sigma*H); n2: the newest
+ * T[m,m-1] (the norm of the next Lanczos vector).  Writes
+ *   out[0] = |expd[m,0]|, out[1] = |expd[m+1,0]|   (expd = exp of the
+ *            Expokit extended matrix, the err1 / err2/n terms of
+ *            emu_base/math/krylov_exp.py:81-84), and if want_coeffs,
+ *   out[2 .. 2+2m) = exp(T) e_0 and out[2+2m .. 4+2m) = expd[m,0] (complex,
+ *            interleaved): the combination coefficients of krylov_exp.py:88.
+ * Replaces torch.linalg.matrix_exp of the (j+3)x(j+3) extended matrix
+ * (krylov_exp.py:77-78) for the Hermitian Lanczos path: O(m^2) per call
+ * (O(m^3) with coefficients) instead of O(m^3). */
+int emu_sv_lanczos_exp(int m, const double *tdiag, const double *tsub,
+                       double sigma_re, double sigma_im, double n2,
+                       int want_coeffs, double *out);
 
 #ifdef __cplusplus
 }

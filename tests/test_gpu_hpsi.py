@@ -27,7 +27,8 @@ def _params(n, with_phase, seed):
     return om, de, ph, u
 
 
-def _hpsi_gpu(n, om, de, ph, u, psi, dtype, sigma=1.0, tile_log=0, min_contig=0):
+def _hpsi_gpu(n, om, de, ph, u, psi, dtype, sigma=1.0, tile_log=0, min_contig=0,
+              kernel="default"):
     from emulators_amd import _native as native
     block, cplx = native.pack_hamiltonian(om.numpy(), de.numpy(), ph.numpy(), u.numpy())
     dev = torch.device("cuda")
@@ -35,7 +36,7 @@ def _hpsi_gpu(n, om, de, ph, u, psi, dtype, sigma=1.0, tile_log=0, min_contig=0)
     v = psi.to(dev, dtype)
     w = torch.empty_like(v)
     native.hpsi(n, params, sigma, v, 1.0, w, complex_coef=cplx, tile_log=tile_log,
-                min_contig=min_contig)
+                min_contig=min_contig, kernel=kernel)
     return w.to("cpu", torch.complex128)
 
 
@@ -63,19 +64,40 @@ def test_hpsi_c64_vs_oracle(gpu_device, n, with_phase):
 
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
 @pytest.mark.parametrize("n,tile_log,min_contig", [
+    (9, 9, 0), (12, 9, 2), (13, 10, 3), (15, 11, 2), (16, 9, 1), (17, 10, 4), (19, 11, 3),
+    (20, 9, 3), (22, 10, 2)])
+@pytest.mark.parametrize("with_phase", [False, True])
+def test_hpsi_wave_kernel_schedules(gpu_device, n, tile_log, min_contig, dtype, with_phase):
+    """Wavefront register tiles (cross-lane flips) over 1-4 passes."""
+    om, de, ph, u = _params(n, with_phase, 4000 + n)
+    psi = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(9 + n))
+    if dtype == torch.complex64:
+        psi = psi.to(torch.complex64).to(torch.complex128)
+    ref = OracleHamiltonian(om, de, ph, u) * psi
+    got = _hpsi_gpu(n, om, de, ph, u, psi, dtype, sigma=0.3 - 0.7j, tile_log=tile_log,
+                    min_contig=min_contig, kernel="wave")
+    ref = (0.3 - 0.7j) * ref
+    tol = 1e-12 if dtype == torch.complex128 else 2e-6
+    err = float((got - ref).norm() / ref.norm())
+    assert err < tol, err
+
+
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+@pytest.mark.parametrize("n,tile_log,min_contig", [
     (9, 8, 0), (10, 8, 2), (12, 8, 3), (13, 8, 1), (14, 9, 4), (16, 8, 2), (17, 12, 3),
     (18, 13, 3), (20, 10, 5), (21, 12, 2)])
 def test_hpsi_multipass_schedules(gpu_device, n, tile_log, min_contig, dtype):
     """Small tiles force 2-4 passes at small n, covering every planner shape."""
     from emulators_amd import _native as native
-    passes = native.plan_passes(n, dtype, tile_log, min_contig)
+    passes = native.plan_passes(n, dtype, tile_log, min_contig, kernel="lds")
     assert passes >= 2
     om, de, ph, u = _params(n, True, 3000 + n)
     psi = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(7 + n))
     if dtype == torch.complex64:
         psi = psi.to(torch.complex64).to(torch.complex128)
     ref = OracleHamiltonian(om, de, ph, u) * psi
-    got = _hpsi_gpu(n, om, de, ph, u, psi, dtype, tile_log=tile_log, min_contig=min_contig)
+    got = _hpsi_gpu(n, om, de, ph, u, psi, dtype, tile_log=tile_log, min_contig=min_contig,
+                    kernel="lds")
     tol = 1e-12 if dtype == torch.complex128 else 2e-6
     err = float((got - ref).norm() / ref.norm())
     assert err < tol, (err, passes)
@@ -93,9 +115,10 @@ def test_hpsi_dense_and_sigma(gpu_device):
     assert torch.allclose(got, ref, atol=1e-12, rtol=0)
 
 
-def test_fused_dots(gpu_device):
+@pytest.mark.parametrize("kernel", ["lds", "wave"])
+def test_fused_dots(gpu_device, kernel):
     from emulators_amd import _native as native
-    n = 11
+    n = 14
     om, de, ph, u = _params(n, True, 5)
     block, cplx = native.pack_hamiltonian(om.numpy(), de.numpy(), ph.numpy(), u.numpy())
     dev = torch.device("cuda")
@@ -106,7 +129,7 @@ def test_fused_dots(gpu_device):
     sv, sp, sigma = 0.7, 1.3, -0.02j
     w = torch.empty(2**n, dtype=torch.complex128, device=dev)
     d = native.hpsi(n, params, sigma, v.to(dev), sv, w, prev=p.to(dev), prev_scale=sp,
-                    dots=True, complex_coef=cplx).cpu().numpy()
+                    dots=True, complex_coef=cplx, kernel=kernel).cpu().numpy()
     ham = OracleHamiltonian(om, de, ph, u)
     wr = sigma * (ham * (sv * v))
     vv, pp = sv * v, sp * p
@@ -173,3 +196,56 @@ def test_correlation(gpu_device, n):
     got = native.correlation(n, psi.to("cuda")).cpu()
     ref = correlation_matrix(psi)
     assert torch.allclose(got, ref, atol=1e-13, rtol=0)
+
+
+@pytest.mark.parametrize("kernel,n", [("lds", 7), ("lds", 13), ("wave", 12), ("wave", 17),
+                                      ("default", 20)])
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_lanczos_matvec(gpu_device, kernel, n, dtype):
+    """z = ws*w - a*sa*pa - b*sb*pb stored into w, r = sigma H z, ||z||^2 and
+    the dots of (z, r, prev = pb)."""
+    from emulators_amd import _native as native
+    om, de, ph, u = _params(n, True, 600 + n)
+    block, cplx = native.pack_hamiltonian(om.numpy(), de.numpy(), ph.numpy(), u.numpy())
+    dev = torch.device("cuda")
+    params = torch.from_numpy(block).to(dev)
+    g = torch.Generator().manual_seed(n)
+    w, pa, pb = (torch.randn(2**n, dtype=torch.complex128, generator=g).to(dtype) for _ in range(3))
+    ws_, sa, sb, a, b, sigma = 0.8, 1.7, 0.6, 0.2 - 0.3j, -0.4 + 0.1j, -0.05j
+    wd, pad, pbd = w.to(dev), pa.to(dev), pb.to(dev)
+    r = torch.empty_like(wd)
+    o = native.lanczos_matvec(n, params, sigma, wd, ws_, pad, sa, a, pbd, sb, b, r,
+                              complex_coef=cplx, kernel=kernel).cpu().numpy()
+    w64, pa64, pb64 = (x.to(torch.complex128) for x in (w, pa, pb))
+    z = ws_ * w64 - a * sa * pa64 - b * sb * pb64
+    tol = 1e-12 if dtype == torch.complex128 else 3e-6
+    assert float((wd.cpu().to(torch.complex128) - z).norm() / z.norm()) < tol
+    zq = wd.cpu().to(torch.complex128)  # the stored (rounded) z feeds the matvec
+    rr = sigma * (OracleHamiltonian(om, de, ph, u) * zq)
+    assert float((r.cpu().to(torch.complex128) - rr).norm() / rr.norm()) < tol * 2
+    assert math.isclose(o[0], float(zq.norm() ** 2), rel_tol=tol)
+    assert math.isclose(o[1], float(rr.norm() ** 2), rel_tol=tol * 10)
+    pbs = sb * pb64
+    for got, ref, scale in [(complex(o[2], o[3]), complex(torch.vdot(zq, rr)), zq.norm() * rr.norm()),
+                            (complex(o[4], o[5]), complex(torch.vdot(pbs, rr)), pbs.norm() * rr.norm()),
+                            (complex(o[6], o[7]), complex(torch.vdot(zq, pbs)), zq.norm() * pbs.norm())]:
+        assert abs(got - ref) < tol * 10 * float(scale)
+
+
+def test_reductions_across_sizes_share_one_workspace(gpu_device):
+    """Regression: reductions of different sizes on one cached workspace.  The
+    finalize counter lives at a fixed offset, so a workspace grown for a large
+    n still gives correct sums for a small n and back."""
+    from emulators_amd import _native as native
+    dev = torch.device("cuda")
+    for n in (18, 10, 18, 9, 12):
+        om, de, ph, u = _params(n, False, 50 + n)
+        block, cplx = native.pack_hamiltonian(om.numpy(), de.numpy(), ph.numpy(), u.numpy())
+        params = torch.from_numpy(block).to(dev)
+        v = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(n))
+        w = torch.empty(2**n, dtype=torch.complex128, device=dev)
+        d = native.hpsi(n, params, 1.0, v.to(dev), 1.0, w, dots=True, complex_coef=cplx).cpu()
+        hv = OracleHamiltonian(om, de, ph, u) * v
+        assert math.isclose(float(d[0]), float(hv.norm() ** 2), rel_tol=1e-12), n
+        dd = native.dot(n, v.to(dev), hv.to(dev)).cpu()
+        assert math.isclose(float(dd[0]), float(v.norm() ** 2), rel_tol=1e-12), n

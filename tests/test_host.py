@@ -117,7 +117,7 @@ def test_planner_covers_every_bit(dtype):
     from emulators_amd import _native as native
     for n in range(1, 33):
         for tl, cmin in [(8, 1), (10, 3), (12, 2), (13, 3)]:
-            p = native.plan_passes(n, dtype, tl, cmin)
+            p = native.plan_passes(n, dtype, tl, cmin, kernel="lds")
             if n <= tl:
                 assert p == 1
             else:
@@ -299,3 +299,42 @@ def test_backend_refuses_cpu_and_noise():
     seq = workloads.chain10()
     with pytest.raises((ValueError, RuntimeError)):
         SVBackend._run_from_sequence_data(seq, SVConfig(gpu=False, log_level=40))
+
+
+@pytest.mark.parametrize("n,with_phase,dt", [(6, False, 0.05), (8, True, 0.02), (9, False, 0.2)])
+def test_lanczos_exp_matches_dense_extended_expm(n, with_phase, dt):
+    """emu_sv_lanczos_exp on real Lanczos matrices of the oracle: the Expokit
+    terms and the combination coefficients equal those of the dense expm of
+    the extended matrix (krylov_exp.py:77-90), at every iteration."""
+    from emulators_amd import _native as native
+    from oracle import emu_sv_oracle as orc
+    g = torch.Generator().manual_seed(n)
+    om = 10 * torch.randn(n, dtype=torch.float64, generator=g)
+    de = 5 * torch.randn(n, dtype=torch.float64, generator=g)
+    ph = torch.randn(n, dtype=torch.float64, generator=g) if with_phase else torch.zeros(n, dtype=torch.float64)
+    u = torch.triu(30 * torch.rand(n, n, dtype=torch.float64, generator=g), 1)
+    u = u + u.T
+    h = orc.OracleHamiltonian(om, de, ph, u)
+    psi = torch.randn(2**n, dtype=torch.complex128, generator=g)
+    sigma = -1j * dt
+    r = orc.krylov_exp_impl(lambda x: sigma * (h * x), psi.clone(), True, 1e-10, 1e-10)
+    T = r.T.numpy()
+    for m in range(1, r.iteration_count + 1):
+        text = np.zeros((m + 2, m + 2), dtype=np.complex128)
+        text[: m + 1, :m] = T[: m + 1, :m]
+        text[m + 1, m] = 1.0
+        ref = torch.linalg.matrix_exp(torch.from_numpy(text)).numpy()
+        e1, e2, c = native.lanczos_exp(text, m, sigma, T[m, m - 1].real, True)
+        assert abs(e1 - abs(ref[m, 0])) < 1e-13 * max(1.0, abs(ref[m, 0]))
+        assert abs(e2 - abs(ref[m + 1, 0])) < 1e-13 * max(1.0, abs(ref[m + 1, 0]))
+        assert np.max(np.abs(c - ref[: m + 1, 0])) < 1e-12
+
+
+def test_lanczos_exp_single_and_pure_phase():
+    """m = 1 and a diagonal J: exp(T) e_0 = exp(sigma alpha_0); with sigma
+    real the phases are trivial; with beta = 0 the extension entries vanish."""
+    from emulators_amd import _native as native
+    t = np.zeros((3, 3), dtype=np.complex128)
+    t[0, 0] = -0.3j * 2.0
+    e1, e2, c = native.lanczos_exp(t, 1, -0.3j, 0.0, True)
+    assert abs(c[0] - np.exp(-0.3j * 2.0)) < 1e-15 and e1 == 0.0 and e2 == 0.0
