@@ -196,6 +196,14 @@ def main() -> int:
             ms, nbytes, passes, sched = time_matvec(seq, dt, args.hpsi_reps, nsteps // 2)
             log(f"{dt}: Lanczos matvec {ms:.3f} ms, {nbytes / ms / 1e6:.0f} GB/s algorithmic, "
                 f"{passes} passes, {sched}")
+        # plain copy of one complex128 vector (2 * 16 * 2^n bytes): calibrates
+        # the profiler's FETCH_SIZE / WRITE_SIZE counters
+        src = torch.zeros(2**n, dtype=torch.complex128, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(args.hpsi_reps):
+            native.convert(n, src, dst)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.hpsi_reps} copies of {16 * 2**n} bytes")
         return 0
     psi128, wall128, mv128 = timed_run(seq, torch.complex128, steps, args.warmup, tol)
     log(f"c128: {steps} steps in {wall128:.2f} s, {mv128} matvecs")

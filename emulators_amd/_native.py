@@ -196,19 +196,20 @@ def lanczos_matvec(n: int, ham_dev: torch.Tensor, sigma: complex, w: torch.Tenso
                    pb_scale: float, b: complex, r: torch.Tensor, complex_coef: bool = True,
                    tile_log: int = 0, min_contig: int = 0, kernel: str = "default") -> torch.Tensor:
     """z = w_scale w - a pa_scale pa - b pb_scale pb (into w); r = sigma H z.
-    Returns a device slice [||z||^2, 7 dots of (z, r, prev=pb)]."""
+    Returns a device slice [||z||^2, Re/Im <z|pb_s>, Re/Im <z|w_s>, ||r||^2,
+    Re/Im <z|r>] (pb_s = pb_scale pb, w_s = w_scale w)."""
     for t in (w, pb, r) + ((pa,) if pa is not None else ()):
         _require_device_vec(t, "lanczos_matvec")
         if t.dtype != w.dtype or t.numel() != (1 << n):
             raise ValueError("lanczos_matvec: vectors must be 2**n elements of one dtype")
     ws = Workspace.get(w.device, n)
-    out = ws.results[24:32]
+    out = ws.results[24:36]
     flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | KERNELS[kernel]
     check(LIB.emu_sv_lanczos_matvec(n, dtype_code(w.dtype), _ptr(ham_dev), float(sigma.real),
                                     float(sigma.imag), _ptr(w), float(w_scale), _ptr(pa),
                                     float(pa_scale), float(a.real), float(a.imag), _ptr(pb),
                                     float(pb_scale), float(b.real), float(b.imag), _ptr(r),
-                                    _ptr(out[0:1]), _ptr(out[1:8]), _ptr(ws.scratch), tile_log,
+                                    _ptr(out[0:5]), _ptr(out[5:12]), _ptr(ws.scratch), tile_log,
                                     min_contig, flags, _stream(w.device)), "lanczos_matvec")
     return out
 

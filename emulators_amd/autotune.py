@@ -11,9 +11,10 @@ A schedule is (kernel kind, tile_log, min_contig):
 contiguous low bits (c >= min_contig) for coalesced segments.
 
 The first use of a (device, n, dtype, coefficient kind) with n >= 16 times
-every candidate on the fused Lanczos H.psi (two untimed and three timed
-calls each, HIP events on the current stream) and keeps the fastest for the
-rest of the process.  Set EMU_SV_AUTOTUNE=0 to use the library default.
+every candidate on one Lanczos iteration (emu_sv_lanczos_matvec: two
+untimed and three timed calls each, HIP events on the current stream) and
+keeps the fastest for the rest of the process.  Set EMU_SV_AUTOTUNE=0 to use the library default, or
+EMU_SV_SCHEDULE="c128=lds:11:2,c64=lds:11:3" to fix the schedule per type.
 """
 
 from __future__ import annotations
@@ -60,18 +61,22 @@ def _enabled() -> bool:
 
 def time_schedule(n: int, dtype: torch.dtype, params: torch.Tensor, sched: Schedule,
                   complex_coef: bool, vecs, warmup: int = 2, reps: int = 3) -> float:
-    """Average ms of one fused Lanczos H.psi (with prev and reductions)."""
-    v, prev, w = vecs
+    """Average ms of one Lanczos iteration (emu_sv_lanczos_matvec with zero
+    orthogonalisation coefficients: the traffic of a real iteration)."""
+    w, pa, pb, r = vecs
+
+    def call():
+        native.lanczos_matvec(n, params, -0.01j, w, 1.0, pa, 1.0, 0j, pb, 1.0, 0j, r,
+                              complex_coef=complex_coef, **sched.kwargs())
+
     for _ in range(warmup):
-        native.hpsi(n, params, -0.01j, v, 1.0, w, prev=prev, prev_scale=1.0, dots=True,
-                    complex_coef=complex_coef, **sched.kwargs())
-    stream = torch.cuda.current_stream(v.device)
+        call()
+    stream = torch.cuda.current_stream(w.device)
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     start.record(stream)
     for _ in range(reps):
-        native.hpsi(n, params, -0.01j, v, 1.0, w, prev=prev, prev_scale=1.0, dots=True,
-                    complex_coef=complex_coef, **sched.kwargs())
+        call()
     stop.record(stream)
     stop.synchronize()
     return start.elapsed_time(stop) / reps
@@ -81,20 +86,33 @@ def tune(n: int, dtype: torch.dtype, device: torch.device, params: torch.Tensor,
          complex_coef: bool, candidates=None) -> tuple[Schedule, dict]:
     """Time every candidate; returns (best, {schedule: ms})."""
     g = torch.Generator(device=device).manual_seed(1234)
-    v = torch.randn(2**n, dtype=dtype, device=device, generator=g)
-    prev = torch.randn(2**n, dtype=dtype, device=device, generator=g)
-    w = torch.empty_like(v)
+    vecs = tuple(torch.randn(2**n, dtype=dtype, device=device, generator=g) for _ in range(4))
     times = {}
     for sched in candidates or CANDIDATES[dtype]:
-        times[sched] = time_schedule(n, dtype, params, sched, complex_coef, (v, prev, w))
+        times[sched] = time_schedule(n, dtype, params, sched, complex_coef, vecs)
     best = min(times, key=times.get)
-    del v, prev, w
+    del vecs
     return best, times
+
+
+def forced_schedule(dtype: torch.dtype) -> Schedule | None:
+    """The schedule EMU_SV_SCHEDULE fixes for ``dtype``, if any."""
+    spec = os.environ.get("EMU_SV_SCHEDULE", "")
+    key = "c128" if dtype == torch.complex128 else "c64"
+    for item in filter(None, (x.strip() for x in spec.split(","))):
+        name, _, val = item.partition("=")
+        if name.strip() == key:
+            kind, tl, cmin = val.split(":")
+            return Schedule(kind.strip(), int(tl), int(cmin))
+    return None
 
 
 def best_schedule(n: int, dtype: torch.dtype, device: torch.device, params: torch.Tensor,
                   complex_coef: bool) -> Schedule:
     """The tuned schedule for this shape (DEFAULT below MIN_TUNED_QUBITS or when disabled)."""
+    forced = forced_schedule(dtype)
+    if forced is not None:
+        return forced
     if n < MIN_TUNED_QUBITS or not _enabled():
         return DEFAULT
     idx = device.index if device.index is not None else torch.cuda.current_device()

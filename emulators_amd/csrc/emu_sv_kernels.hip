@@ -109,6 +109,7 @@ struct PassArgs {
   void *zout;
   const void *pa, *pb;
   double ca_re, ca_im, cb_re, cb_im;
+  double pb_scale;
   double *partials2;
   double v_scale, prev_scale;
   double sig_re, sig_im;
@@ -123,9 +124,14 @@ struct PassArgs {
 // Load one input amplitude of a pass: v_scale*v[x], or in orthogonalising
 // mode z = v_scale*v[x] - ca*pa[x] - cb*pb[x] (FP64 arithmetic, stored back
 // to v and accumulated into nrm).
+// Reductions of the orthogonalising load: ||z||^2, <z|pb_s>, <z|w_s> with
+// pb_s = pb_scale*pb the current Lanczos vector and w_s = v_scale*v the
+// previous matvec (5 doubles).
+constexpr int kOrthoSums = 5;
+
 template <typename T>
 __device__ __forceinline__ typename Vec2<T>::type load_input(const PassArgs &a, uint64_t x,
-                                                             double &nrm) {
+                                                             double (&osum)[kOrthoSums]) {
   using T2 = typename Vec2<T>::type;
   const T2 v = reinterpret_cast<const T2 *>(a.v)[x];
   T2 z;
@@ -135,21 +141,30 @@ __device__ __forceinline__ typename Vec2<T>::type load_input(const PassArgs &a, 
     z.y = v.y * vs;
     return z;
   }
-  double zr = a.v_scale * static_cast<double>(v.x), zi = a.v_scale * static_cast<double>(v.y);
+  const double wr = a.v_scale * static_cast<double>(v.x), wi = a.v_scale * static_cast<double>(v.y);
+  double zr = wr, zi = wi;
   if (a.pa) {
     const T2 p = reinterpret_cast<const T2 *>(a.pa)[x];
     zr -= a.ca_re * p.x - a.ca_im * p.y;
     zi -= a.ca_re * p.y + a.ca_im * p.x;
   }
+  double br = 0.0, bi = 0.0;
   if (a.pb) {
     const T2 p = reinterpret_cast<const T2 *>(a.pb)[x];
+    br = a.pb_scale * static_cast<double>(p.x);
+    bi = a.pb_scale * static_cast<double>(p.y);
     zr -= a.cb_re * p.x - a.cb_im * p.y;
     zi -= a.cb_re * p.y + a.cb_im * p.x;
   }
   z.x = static_cast<T>(zr);
   z.y = static_cast<T>(zi);
   reinterpret_cast<T2 *>(a.zout)[x] = z;
-  nrm += static_cast<double>(z.x) * z.x + static_cast<double>(z.y) * z.y;
+  const double qr = z.x, qi = z.y;
+  osum[0] += qr * qr + qi * qi;
+  osum[1] += qr * br + qi * bi;
+  osum[2] += qr * bi - qi * br;
+  osum[3] += qr * wr + qi * wi;
+  osum[4] += qr * wi - qi * wr;
   return z;
 }
 
@@ -186,8 +201,8 @@ __global__ __launch_bounds__(kThreads) void hpsi_pass_kernel(const PassArgs a) {
   const double *__restrict__ H = a.ham;
 
   // Stage the tile.
-  double onrm[1] = {0.0};
-  for (uint32_t l = tid; l < tsize; l += kThreads) tile[l] = load_input<T>(a, gidx(l), onrm[0]);
+  double osum[kOrthoSums] = {0, 0, 0, 0, 0};
+  for (uint32_t l = tid; l < tsize; l += kThreads) tile[l] = load_input<T>(a, gidx(l), osum);
   if (a.diag) {
     // Outer-bit constants of this tile, tabulated by diag_prep_kernel.
     const double *row = a.dtab + g * kDiagRow;
@@ -313,7 +328,7 @@ __global__ __launch_bounds__(kThreads) void hpsi_pass_kernel(const PassArgs a) {
   if (DOTS) block_reduce_store<7>(dots, s_red, a.partials + (uint64_t)blockIdx.x * kNumDots);
   if (a.ortho) {
     __syncthreads();
-    block_reduce_store<1>(onrm, s_red, a.partials2 + (uint64_t)blockIdx.x * kNumDots);
+    block_reduce_store<kOrthoSums>(osum, s_red, a.partials2 + (uint64_t)blockIdx.x * kNumDots);
   }
 }
 
@@ -396,9 +411,9 @@ __global__ __launch_bounds__(kThreads) void hpsi_wave_kernel(const PassArgs a) {
 
   // Load the tile: slot r of lane holds local index lane | r << 6.
   T2 psi[R];
-  double onrm = 0.0;
+  double osum[kOrthoSums] = {0, 0, 0, 0, 0};
 #pragma unroll
-  for (int r = 0; r < R; ++r) psi[r] = load_input<T>(a, gidx((uint32_t)lane | ((uint32_t)r << 6)), onrm);
+  for (int r = 0; r < R; ++r) psi[r] = load_input<T>(a, gidx((uint32_t)lane | ((uint32_t)r << 6)), osum);
 
   // Flip coefficients per local bit (uniform), conj for a clear bit.
   uint32_t fm = 0;
@@ -518,8 +533,11 @@ __global__ __launch_bounds__(kThreads) void hpsi_wave_kernel(const PassArgs a) {
     }
   }
   if (a.ortho) {
-    const double sm = wave_sum(onrm);
-    if (lane == 0) a.partials2[g * kNumDots] = sm;
+#pragma unroll
+    for (int k = 0; k < kOrthoSums; ++k) {
+      const double sm = wave_sum(osum[k]);
+      if (lane == 0) a.partials2[g * kNumDots + k] = sm;
+    }
   }
 }
 
@@ -588,13 +606,17 @@ __global__ __launch_bounds__(kThreads) void finalize2_kernel(const double *rows,
   __syncthreads();
   if (!last) return;
   __threadfence();
-  if (threadIdx.x < K) {
-    double sm = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b)
-      sm += __hip_atomic_load(&stage[b * kNumDots + threadIdx.x], __ATOMIC_RELAXED,
-                              __HIP_MEMORY_SCOPE_AGENT);
-    out[threadIdx.x] = sm;
+  // same (row lane, value) layout over the stage rows, then the same tree
+  double sm = 0.0;
+  for (unsigned b = rl; b < gridDim.x; b += kThreads / 8)
+    sm += __hip_atomic_load(&stage[b * kNumDots + v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  red[threadIdx.x] = sm;
+  __syncthreads();
+  for (int hlf = kThreads / 2; hlf >= 8; hlf >>= 1) {
+    if (threadIdx.x < hlf) red[threadIdx.x] += red[threadIdx.x + hlf];
+    __syncthreads();
   }
+  if (threadIdx.x < K) out[threadIdx.x] = red[threadIdx.x];
   if (threadIdx.x == 0) atomicExch(ticket, 0u);
 }
 
@@ -1008,8 +1030,8 @@ int launch_wave_any(const PassArgs &a, bool cplx, bool dots, hipStream_t st) {
 
 struct OrthoIn {
   const void *pa, *pb;
-  double ca_re, ca_im, cb_re, cb_im;
-  double *norm2_out;
+  double ca_re, ca_im, cb_re, cb_im, pb_scale;
+  double *norm2_out;  // kOrthoSums doubles
 };
 
 template <typename T>
@@ -1043,6 +1065,7 @@ int run_hpsi(int n, const double *ham, double sr, double si, const void *v, doub
     a.ca_im = ortho ? ortho->ca_im : 0.0;
     a.cb_re = ortho ? ortho->cb_re : 0.0;
     a.cb_im = ortho ? ortho->cb_im : 0.0;
+    a.pb_scale = ortho ? ortho->pb_scale : 0.0;
     a.partials2 = partials2_ptr(ws, n);
     a.v_scale = (ortho != nullptr && p > 0) ? 1.0 : vsc;
     a.prev_scale = psc;
@@ -1075,7 +1098,7 @@ int run_hpsi(int n, const double *ham, double sr, double si, const void *v, doub
     const int t = a.c + a.m;
     const int nrow = (int)(1ull << (n - t));
     if (a.ortho) {
-      rc = finalize(a.partials2, nrow, 1, ws, n, ortho->norm2_out, st);
+      rc = finalize(a.partials2, nrow, kOrthoSums, ws, n, ortho->norm2_out, st);
       if (rc) return rc;
     }
     if (fuse) {
@@ -1263,12 +1286,13 @@ int emu_sv_lanczos_matvec(int n, int dtype, const double *ham, double sigma_re, 
   o.ca_im = a_im * pa_scale;
   o.cb_re = b_re * pb_scale;
   o.cb_im = b_im * pb_scale;
+  o.pb_scale = pb_scale;
   o.norm2_out = norm2_out;
   const bool cplx = (flags & EMU_SV_FLAG_COMPLEX) != 0;
   if (dtype == EMU_SV_C128)
-    return run_hpsi<double>(n, ham, sigma_re, sigma_im, w, w_scale, pb, pb_scale, r, dots_out,
+    return run_hpsi<double>(n, ham, sigma_re, sigma_im, w, w_scale, nullptr, 0.0, r, dots_out,
                             workspace, kind, tile_log, min_contig, cplx, stream, &o);
-  return run_hpsi<float>(n, ham, sigma_re, sigma_im, w, w_scale, pb, pb_scale, r, dots_out,
+  return run_hpsi<float>(n, ham, sigma_re, sigma_im, w, w_scale, nullptr, 0.0, r, dots_out,
                          workspace, kind, tile_log, min_contig, cplx, stream, &o);
 }
 

@@ -14,9 +14,10 @@ Two implementations sit behind it:
   fused engine.  Each Lanczos iteration is one H.psi launch sequence
   (tiled passes, the diagonal generated on the fly): its first pass
   orthogonalises the previous iteration's vector,
-  z = w - T[j-1,j] v_{j-1} - T[j,j] v_j, reducing ||z||^2, and its last pass
-  reduces ||Hz||^2, <z|Hz>, <v_j|Hz> and <z|v_j>; one read-back per
-  iteration.  Basis vectors are stored unnormalised with a host-side scale
+  z = w - T[j-1,j] v_{j-1} - T[j,j] v_j, reducing ||z||^2, <z|v_j> and
+  <z|w>, and its last pass reduces ||Hz||^2 and <z|Hz>.  With H Hermitian,
+  T[j,j+1] = <v_j|sigma H v_{j+1}> = sigma conj(<v_{j+1}|sigma H v_j>)/conj(sigma),
+  so the previous vector is not read again; one read-back per iteration.  Basis vectors are stored unnormalised with a host-side scale
   (the normalisation is folded into the next launch).  T is complex128 on
   the host; its exponential and the Expokit error terms come from the
   symmetric tridiagonal form (emu_sv_lanczos_exp: T = W^-1 sigma J W with J
@@ -161,12 +162,13 @@ def _fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
         if err < exp_tolerance:
             _, _, coeffs = native.lanczos_exp(T, j + 1, op.sigma, n2, True)
             return KrylovExpResult(combine(coeffs), True, False, j + 1)
-        # dots of w'_{j+1} = r / n2 against v_{j+1} = z / n2 and v_j
+        # coefficients of iteration j+1: w'_{j+1} = r/n2, v_{j+1} = z/n2,
+        # <v_j|w'_{j+1}> = sigma conj(<v_{j+1}|w'_j>) / conj(sigma) (H Hermitian)
         wbuf, w_scale = r, 1.0 / n2
-        ww = o[1] / (n2 * n2)
-        vw = complex(o[2], o[3]) / (n2 * n2)
-        pw = complex(o[4], o[5]) / n2
-        vp = complex(o[6], o[7]) / n2
+        vp = complex(o[1], o[2]) / n2
+        pw = op.sigma * complex(o[3], o[4]).conjugate() / (n2 * op.sigma.conjugate())
+        ww = o[5] / (n2 * n2)
+        vw = complex(o[6], o[7]) / (n2 * n2)
     _, _, coeffs = native.lanczos_exp(T, max_krylov_dim, op.sigma, T[max_krylov_dim, max_krylov_dim - 1].real, True)
     return KrylovExpResult(combine(coeffs), False, False, max_krylov_dim)
 

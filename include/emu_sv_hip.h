@@ -109,12 +109,16 @@ int emu_sv_hpsi(int n, int dtype, const double *ham, double sigma_re,
 /* One fused Lanczos iteration step (orthogonalise, then the next matvec):
  *   z = w_scale*w - a*(pa_scale*pa) - b*(pb_scale*pb)   (stored back into w)
  *   r = sigma*H*z
- * norm2_out[0] = ||z||^2; dots_out[0..6] as emu_sv_hpsi with v = z and
- * prev = pb (scale pb_scale).  The orthogonalisation of iteration j
- * (emu_base/math/krylov_exp.py:54-61, two-vector MGS with coefficients from
- * the previous call's dots) runs inside the first H.psi pass of iteration
- * j+1 (krylov_exp.py:50-52), so each Lanczos iteration is one call and one
- * host read-back.  w, r, pa, pb must be distinct; pa may be NULL. */
+ * norm2_out[0..4] = ||z||^2, <z|pb_s>, <z|w_s> (complex as re, im) with
+ *   pb_s = pb_scale*pb and w_s = w_scale*w (the input before the update);
+ * dots_out[0..2] = ||r||^2, <z|r>.
+ * With H Hermitian, <v_j|sigma H v_{j+1}> = sigma conj(<z|w_s>)/(|z| conj(sigma))
+ * for v_{j+1} = z/|z|, w_s = sigma H v_j, v_j = pb_s: so every Lanczos
+ * coefficient of the next iteration (krylov_exp.py:54-61, two-vector MGS)
+ * comes from this call, whose first H.psi pass performs the
+ * orthogonalisation of krylov_exp.py:56-61 and whose passes are the matvec of
+ * krylov_exp.py:50.  One call and one host read-back per Lanczos iteration.
+ * w, r, pa, pb must be distinct; pa may be NULL. */
 int emu_sv_lanczos_matvec(int n, int dtype, const double *ham, double sigma_re,
                           double sigma_im, void *w, double w_scale,
                           const void *pa, double pa_scale, double a_re,

@@ -338,3 +338,12 @@ def test_lanczos_exp_single_and_pure_phase():
     t[0, 0] = -0.3j * 2.0
     e1, e2, c = native.lanczos_exp(t, 1, -0.3j, 0.0, True)
     assert abs(c[0] - np.exp(-0.3j * 2.0)) < 1e-15 and e1 == 0.0 and e2 == 0.0
+
+
+def test_forced_schedule_env(monkeypatch):
+    from emulators_amd import autotune
+    monkeypatch.setenv("EMU_SV_SCHEDULE", "c128=lds:11:2, c64=wave:10:3")
+    assert autotune.forced_schedule(torch.complex128) == autotune.Schedule("lds", 11, 2)
+    assert autotune.forced_schedule(torch.complex64) == autotune.Schedule("wave", 10, 3)
+    monkeypatch.delenv("EMU_SV_SCHEDULE")
+    assert autotune.forced_schedule(torch.complex64) is None
