@@ -113,6 +113,22 @@ def time_matvec(seq, dtype, reps, step):
     return ms, alg_bytes, passes, sched
 
 
+def measured_traffic(tag, sched, n):
+    """HBM bytes per Lanczos iteration from the committed rocprofv3 counter
+    runs, when they were taken on this schedule and size (else None)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "lanczos_matvec_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    ent = rec.get(tag)
+    if not ent or rec.get("n_qubits") != n or ent.get("schedule") != sched.kwargs():
+        return None
+    return float(ent["bytes_per_call"])
+
+
 def cpu_baseline(seq, mean_iters, step, sample_iters):
     """Oracle (torch CPU, complex128) on the N=26 Hamiltonian of one step:
     the diagonal build plus ``sample_iters`` Lanczos iterations, scaled to
@@ -129,6 +145,7 @@ def cpu_baseline(seq, mean_iters, step, sample_iters):
     ham = orc.OracleHamiltonian(seq.omega[step], seq.delta[step], seq.phi[step],
                                 seq.interaction_matrix(0.0)[0])
     t_diag = time.perf_counter() - t0
+    log(f"CPU baseline: diagonal built in {t_diag:.1f} s")
     v = torch.zeros(2**n, dtype=torch.complex128)
     v[0] = 1.0
     prev = None
@@ -144,6 +161,7 @@ def cpu_baseline(seq, mean_iters, step, sample_iters):
         nrm = w.norm()
         w /= nrm
         prev, v = v, w
+        log(f"CPU baseline: Lanczos iteration done at {time.perf_counter() - t0:.1f} s")
     t_iter = (time.perf_counter() - t0) / sample_iters
     t_step = t_diag + mean_iters * t_iter
     return {
@@ -164,7 +182,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--hpsi-reps", type=int, default=20)
-    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c64", action="store_true")
     ap.add_argument("--duration", type=int, default=3000, help="pulse duration in ns")
@@ -270,7 +288,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": gbs128 / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic("c128", sched128, n),
             "kernel": f"one Lanczos iteration (emu_sv_lanczos_matvec, complex128): "
                       f"orthogonalise + {passes128} H.psi passes ({sched128.kernel} kernel, "
                       f"tile 2^{sched128.tile_log}) + reductions; algorithmic bytes 3*D*16 per call "
