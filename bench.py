@@ -176,6 +176,72 @@ def cpu_baseline(seq, mean_iters, step, sample_iters):
     }
 
 
+def adjoint_cost(reps: int = 2) -> dict:
+    """The 16-atom gradient workload: 16 atoms at seeded random positions
+    (30x30 um box, min spacing 5 um), 100 piecewise-constant 10 ns steps with
+    global Omega_k ~ U(0, 4 pi), delta_k ~ U(-10, 10) (200 parameters), loss =
+    final energy + sum of occupations.  Times one forward run (100 Krylov
+    steps) against one full gradient (forward with checkpoints every 10 steps
+    + hand-written adjoint sweep), best of ``reps`` after one warmup each."""
+    import torch
+
+    from emulators_amd import workloads
+    from emulators_amd.emu_base.math.krylov_exp import ScaledHamiltonian, krylov_exp
+    from emulators_amd.emu_sv import RydbergHamiltonian
+    from emulators_amd.emu_sv.adjoint import EnergyExpectation, evolve_pulse, total_occupation
+
+    pos, om_np, de_np = workloads.random16_parameters()
+    n, s = pos.shape[0], len(om_np)
+    u = torch.as_tensor(workloads.interaction_matrix_from_positions(pos, workloads.C6))
+    tol = 1e-10
+    zeros = torch.zeros(n, dtype=torch.float64)
+
+    def psi0():
+        p = torch.zeros(2**n, dtype=torch.complex128, device="cuda")
+        p[0] = 1.0
+        return p
+
+    def forward():
+        psi = psi0()
+        for k in range(s):
+            ham = RydbergHamiltonian(torch.full((n,), float(om_np[k])), torch.full((n,), float(de_np[k])),
+                                     zeros, u, device="cuda")
+            psi = krylov_exp(ScaledHamiltonian(ham, -0.01j), psi, tol, tol)
+        torch.cuda.synchronize()
+
+    def gradient(mode):
+        om = torch.as_tensor(om_np).clone().requires_grad_(True)
+        de = torch.as_tensor(de_np).clone().requires_grad_(True)
+        omegas, deltas = om[:, None].expand(s, n), de[:, None].expand(s, n)
+        phis = torch.zeros(s, n, dtype=torch.float64)
+        psi = evolve_pulse(omegas, deltas, phis, u, psi0(), torch.full((s,), 0.01), tol, 10, 100, mode)
+        loss = EnergyExpectation.apply(omegas[-1], deltas[-1], phis[-1], u, psi) + total_occupation(psi)
+        loss.backward()
+        torch.cuda.synchronize()
+        return om.grad, de.grad
+
+    def best(fn, *a):
+        fn(*a)
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn(*a)
+            times.append(time.perf_counter() - t0)
+        return min(times)
+
+    t_fwd = best(forward)
+    out = {"workload": "random16: N=16, 100 steps x 10 ns, global Omega_k/delta_k (200 parameters), "
+                       "loss = E(T) + sum_i <n_i>(T), complex128, tolerance 1e-10, checkpoints every 10",
+           "forward_s": t_fwd}
+    for mode in ("reverse", "recompute"):
+        t = best(gradient, mode)
+        out[mode] = {"gradient_s": t, "ratio_vs_forward": t / t_fwd}
+    log(f"adjoint: forward {t_fwd:.3f} s, gradient {out['reverse']['gradient_s']:.3f} s "
+        f"(x{out['reverse']['ratio_vs_forward']:.2f}, reverse), "
+        f"{out['recompute']['gradient_s']:.3f} s (x{out['recompute']['ratio_vs_forward']:.2f}, recompute)")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +252,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c64", action="store_true")
     ap.add_argument("--duration", type=int, default=3000, help="pulse duration in ns")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--only-adjoint", action="store_true", help="time only the gradient workload")
     ap.add_argument("--only-matvec", action="store_true",
                     help="time only the Lanczos matvec (for profiler runs)")
     args = ap.parse_args()
@@ -209,6 +277,9 @@ def main() -> int:
     n = seq.omega.shape[1]
     log(f"workload triangular26: N={n}, {nsteps} steps; timing {steps} after {args.warmup} warmup")
 
+    if args.only_adjoint:
+        print(json.dumps(adjoint_cost()), flush=True)
+        return 0
     if args.only_matvec:
         for dt in (torch.complex128, torch.complex64):
             ms, nbytes, passes, sched = time_matvec(seq, dt, args.hpsi_reps, nsteps // 2)
@@ -254,6 +325,7 @@ def main() -> int:
                        "passes": passes64, "schedule": sched64.kwargs()}
     log(f"Lanczos matvec c128 {ms128:.3f} ms ({gbs128:.0f} GB/s algorithmic)")
 
+    adjoint = None if args.no_adjoint else adjoint_cost()
     mean_iters = mv128 / steps
     cpu = None
     if not args.no_cpu_baseline:
@@ -297,6 +369,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "c64": result64,
         "hpsi": hpsi,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
     return 0

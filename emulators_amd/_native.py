@@ -50,6 +50,7 @@ SIGNATURES = {
     "emu_sv_scale": (_i, [_i, _i, _vp, _d, _vp]),
     "emu_sv_convert": (_i, [_i, _i, _vp, _i, _vp, _vp]),
     "emu_sv_correlation": (_i, [_i, _i, _vp, _vp, _vp, _vp]),
+    "emu_sv_parameter_traces": (_i, [_i, _i, _i, _vp, _vp, ctypes.c_uint64, _i, _vp, _vp, _vp]),
     "emu_sv_expm": (_i, [_i, _dp, _dp]),
     "emu_sv_lanczos_exp": (_i, [_i, _dp, _dp, _d, _d, _d, _i, _dp]),
 }
@@ -275,6 +276,42 @@ def correlation(n: int, psi: torch.Tensor) -> torch.Tensor:
     check(LIB.emu_sv_correlation(n, dtype_code(psi.dtype), _ptr(psi), _ptr(out), _ptr(ws.scratch),
                                  _stream(psi.device)), "correlation")
     return out
+
+
+class Traces:
+    """Per-qubit sums of <W|s+_q|Z>, <W|s-_q|Z>, <W|n_q|Z> and <W|n_q n_q'|Z>
+    (complex128 numpy, QUBIT order; ``pair`` is upper-triangular or None)."""
+
+    def __init__(self, up, dn, occ, pair):
+        self.up, self.dn, self.occ, self.pair = up, dn, occ, pair
+
+
+def parameter_traces(n: int, w: torch.Tensor, z: torch.Tensor, pairs: bool = False) -> Traces:
+    """Traces of sum_l <W_l| . |Z_l> for rows W_l, Z_l of two (count, 2**n)
+    (or (2**n,)) device tensors of one complex dtype (emu_sv_parameter_traces)."""
+    w2 = w.reshape(-1, w.shape[-1]) if w.dim() > 1 else w.reshape(1, -1)
+    z2 = z.reshape(-1, z.shape[-1]) if z.dim() > 1 else z.reshape(1, -1)
+    if w2.shape != z2.shape or w2.dtype != z2.dtype or w2.shape[1] != (1 << n):
+        raise ValueError("parameter_traces: W and Z must be (count, 2**n) of one dtype")
+    if not (w2.is_cuda and z2.is_cuda):
+        raise ValueError("parameter_traces: the emu_sv engine needs device (GPU) tensors")
+    w2 = w2.contiguous()
+    z2 = z2.contiguous()
+    ws = Workspace.get(w2.device, n)
+    nout = 6 * n + (2 * n * n if pairs else 0)
+    out = torch.zeros(nout, dtype=torch.float64, device=w2.device)
+    check(LIB.emu_sv_parameter_traces(n, dtype_code(w2.dtype), w2.shape[0], _ptr(w2), _ptr(z2),
+                                      ctypes.c_uint64(1 << n), int(pairs), _ptr(out),
+                                      _ptr(ws.scratch), _stream(w2.device)), "parameter_traces")
+    o = out.cpu().numpy()
+    c = o[: 6 * n].view(np.complex128)
+    rev = slice(None, None, -1)  # bit position b <-> qubit n-1-b
+    up, dn, occ = c[:n][rev].copy(), c[n: 2 * n][rev].copy(), c[2 * n: 3 * n][rev].copy()
+    pair = None
+    if pairs:
+        pb = o[6 * n:].view(np.complex128).reshape(n, n)  # [b][b'] for b < b'
+        pair = pb[rev][:, rev].T.copy()  # qubit (i, j) = bits (n-1-i, n-1-j): i < j  <=>  b > b'
+    return Traces(up, dn, occ, pair)
 
 
 def expm(a: np.ndarray) -> np.ndarray:

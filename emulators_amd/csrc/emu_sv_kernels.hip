@@ -933,6 +933,120 @@ __global__ void corr_finalize_kernel(int n, const double *partials, int nblk, do
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Parameter traces: per-bit sums of conj(W_l) Z_l over flipped / masked
+// index sets (the per-qubit traces of every Hamiltonian-parameter gradient).
+// Grid (gx, n): block row b owns bit b; its blocks stride over chunks of
+// kTraceChunk elements.  The flip partner Z_l(x ^ 2^b) is a second read of
+// Z, served from L2/MALL since the n block rows of a chunk run together.
+// ---------------------------------------------------------------------------
+constexpr int kTraceChunk = 1024;
+constexpr int kTraceBlocks = 2048;
+
+template <int K>
+__device__ __forceinline__ void block_reduce_partial(double (&v)[K], int nk, double *red, double *out) {
+  // red: kWaves*K doubles of LDS; sums the first nk entries of v over the block
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (k < nk) {
+      const double sm = wave_sum(v[k]);
+      if (lane == 0) red[wave * K + k] = sm;
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nk; k += kThreads) {
+    double sm = 0.0;
+    for (int w = 0; w < kWaves; ++w) sm += red[w * K + k];
+    out[k] = sm;
+  }
+}
+
+template <typename T, bool PAIRS>
+__global__ __launch_bounds__(kThreads) void traces_kernel(int n, int count, const typename Vec2<T>::type *W,
+                                                          const typename Vec2<T>::type *Z, uint64_t stride,
+                                                          double *partials) {
+  using T2 = typename Vec2<T>::type;
+  constexpr int K = PAIRS ? 6 + 2 * EMU_SV_MAX_QUBITS : 6;
+  __shared__ double red[kWaves * K];
+  const int b = blockIdx.y;
+  const uint64_t D = 1ull << n;
+  const uint64_t ch = D < (uint64_t)kTraceChunk ? D : (uint64_t)kTraceChunk;
+  const uint64_t nchunks = D / ch;
+  const uint64_t fb = 1ull << b;
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = 0.0;
+  for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    for (uint64_t off = threadIdx.x; off < ch; off += kThreads) {
+      const uint64_t x = c * ch + off;
+      const bool set = (x >> b) & 1;
+      for (int l = 0; l < count; ++l) {
+        const uint64_t row = (uint64_t)l * stride;
+        const T2 w = W[row + x];
+        const T2 zz = Z[row + x];
+        const T2 zp = Z[row + (x ^ fb)];
+        const double wr = w.x, wi = w.y;
+        // conj(w) * zp
+        const double fr = wr * (double)zp.x + wi * (double)zp.y;
+        const double fi = wr * (double)zp.y - wi * (double)zp.x;
+        if (set) {
+          const double pr = wr * (double)zz.x + wi * (double)zz.y;
+          const double pi = wr * (double)zz.y - wi * (double)zz.x;
+          acc[0] += fr;
+          acc[1] += fi;
+          acc[4] += pr;
+          acc[5] += pi;
+          if (PAIRS) {
+#pragma unroll
+            for (int b2 = 0; b2 < EMU_SV_MAX_QUBITS; ++b2) {
+              if (b2 > b && b2 < n && ((x >> b2) & 1)) {
+                acc[6 + 2 * b2] += pr;
+                acc[7 + 2 * b2] += pi;
+              }
+            }
+          }
+        } else {
+          acc[2] += fr;
+          acc[3] += fi;
+        }
+      }
+    }
+  }
+  const int nk = PAIRS ? 6 + 2 * n : 6;
+  block_reduce_partial<K>(acc, nk, red,
+                          partials + ((uint64_t)b * gridDim.x + blockIdx.x) * (6 + 2 * EMU_SV_MAX_QUBITS));
+}
+
+// out: up[n], dn[n], occ[n], pair[n*n] complex; partials [b][gx][6 + 2*32].
+__global__ __launch_bounds__(kThreads) void traces_finalize_kernel(int n, int gx, int pairs,
+                                                                   const double *partials, double *out) {
+  const int per = 6 + 2 * EMU_SV_MAX_QUBITS;
+  const int nout = pairs ? 6 * n + 2 * n * n : 6 * n;
+  for (int e = blockIdx.x * kThreads + threadIdx.x; e < nout; e += gridDim.x * kThreads) {
+    int b, k;
+    if (e < 6 * n) {
+      const int q = e / (2 * n);          // 0 up, 1 dn, 2 occ
+      const int r = e % (2 * n);
+      b = r / 2;
+      k = 2 * q + (r & 1);
+    } else {
+      const int r = e - 6 * n;            // pair: (b * n + b2) * 2 + re/im
+      b = r / (2 * n);
+      const int b2 = (r / 2) % n;
+      if (b2 <= b) {
+        out[e] = 0.0;
+        continue;
+      }
+      k = 6 + 2 * b2 + (r & 1);
+    }
+    double sm = 0.0;
+    for (int g = 0; g < gx; ++g) sm += partials[((uint64_t)b * gx + g) * per + k];
+    out[e] = sm;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Planner
 // ---------------------------------------------------------------------------
@@ -1423,6 +1537,45 @@ int emu_sv_correlation(int n, int dtype, const void *psi, double *corr_out, void
   hipLaunchKernelGGL(corr_finalize_kernel, dim3(1), dim3(kThreads), 0, stream, n, part, nb,
                      corr_out);
   return hip_check(hipGetLastError(), "correlation finalize launch");
+}
+
+int emu_sv_parameter_traces(int n, int dtype, int count, const void *w, const void *z,
+                            uint64_t stride, int want_pairs, double *out, void *workspace,
+                            hipStream_t stream) {
+  if (int rc = check_n(n)) return rc;
+  if (int rc = check_dtype(dtype)) return rc;
+  if (count < 1) return fail(EMU_SV_EINVAL, "parameter_traces: count must be >= 1");
+  if (!w || !z || !out || !workspace) return fail(EMU_SV_EINVAL, "parameter_traces: null pointer");
+  if (count > 1 && stride < (1ull << n))
+    return fail(EMU_SV_EINVAL, "parameter_traces: rows overlap (stride < 2^n)");
+  const uint64_t D = 1ull << n;
+  const uint64_t ch = D < (uint64_t)kTraceChunk ? D : (uint64_t)kTraceChunk;
+  const uint64_t nchunks = D / ch;
+  uint64_t gx = (uint64_t)(kTraceBlocks / n);
+  if (gx > nchunks) gx = nchunks;
+  if (gx < 1) gx = 1;
+  double *part = partials_ptr(workspace, n);
+  const dim3 grid((unsigned)gx, (unsigned)n);
+  if (dtype == EMU_SV_C128) {
+    if (want_pairs)
+      hipLaunchKernelGGL((traces_kernel<double, true>), grid, dim3(kThreads), 0, stream, n, count,
+                         (const double2 *)w, (const double2 *)z, stride, part);
+    else
+      hipLaunchKernelGGL((traces_kernel<double, false>), grid, dim3(kThreads), 0, stream, n, count,
+                         (const double2 *)w, (const double2 *)z, stride, part);
+  } else {
+    if (want_pairs)
+      hipLaunchKernelGGL((traces_kernel<float, true>), grid, dim3(kThreads), 0, stream, n, count,
+                         (const float2 *)w, (const float2 *)z, stride, part);
+    else
+      hipLaunchKernelGGL((traces_kernel<float, false>), grid, dim3(kThreads), 0, stream, n, count,
+                         (const float2 *)w, (const float2 *)z, stride, part);
+  }
+  if (int rc = hip_check(hipGetLastError(), "parameter_traces launch")) return rc;
+  const int nout = want_pairs ? 6 * n + 2 * n * n : 6 * n;
+  hipLaunchKernelGGL(traces_finalize_kernel, dim3((nout + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                     stream, n, (int)gx, want_pairs, (const double *)part, out);
+  return hip_check(hipGetLastError(), "parameter_traces finalize launch");
 }
 
 }  // extern "C"

@@ -93,8 +93,56 @@ def _error_estimate(expd: np.ndarray, j: int, n: float) -> float:
     return err1 if err1 < err2 else (err1 * err2 / (err1 - err2))
 
 
+@dataclass
+class LanczosRun:
+    """Lanczos decomposition of one fused run: op(V) ~ V T, v = initial_norm * V e_0.
+
+    ``basis[k] * scales[k]`` is the k-th orthonormal vector;
+    ``T`` the m x m Lanczos matrix (complex128, op = sigma H) and ``n2`` the
+    norm of the next vector (0 after a happy breakdown);
+    ``coeffs`` = exp(T_ext) e_0 over the m (+1) vectors of the corrected
+    Krylov approximation, exp(op) v = initial_norm * sum_k coeffs_k V_k."""
+
+    basis: list
+    scales: list
+    T: np.ndarray
+    coeffs: np.ndarray
+    initial_norm: float
+    converged: bool
+    happy_breakdown: bool
+    iteration_count: int
+    n2: float
+
+    @property
+    def m(self) -> int:
+        return self.T.shape[0]
+
+    def extended(self) -> np.ndarray:
+        """The matrix whose exponential gives ``coeffs``: T with the row of the
+        next vector appended ((m+1) x (m+1), last column zero) unless the
+        iteration broke down."""
+        if self.happy_breakdown:
+            return self.T.copy()
+        m = self.m
+        t = np.zeros((m + 1, m + 1), dtype=np.complex128)
+        t[:m, :m] = self.T
+        t[m, m - 1] = self.n2
+        return t
+
+    def combine(self, coeffs: np.ndarray | None = None, out: torch.Tensor | None = None) -> torch.Tensor:
+        """initial_norm * sum_k coeffs_k V_k (default coeffs: exp(T) e_0)."""
+        c = self.coeffs if coeffs is None else np.asarray(coeffs)
+        k = len(c)
+        n = native.nqubits_of(self.basis[0])
+        cc = self.initial_norm * c * np.asarray(self.scales[:k])
+        if out is None:
+            out = torch.empty_like(self.basis[0])
+        return native.lincomb(n, self.basis[:k], cc, out)
+
+
 def _fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
-                   norm_tolerance: float, max_krylov_dim: int) -> KrylovExpResult:
+                   norm_tolerance: float, max_krylov_dim: int,
+                   rows: torch.Tensor | None = None) -> LanczosRun:
     """Lanczos of krylov_exp.py:23-106 with one fused launch sequence and one
     read-back per iteration.
 
@@ -103,7 +151,10 @@ def _fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
     vector, unnormalised) and then r = sigma H z_j, with ||z_j||^2 and the
     dots of r that give T[j,j+1] and T[j+1,j+1].  The convergence test of
     iteration j therefore lands after the matvec of iteration j+1: one matvec
-    per Krylov step is computed and not used."""
+    per Krylov step is computed and not used.
+
+    ``rows``: optional (max_krylov_dim + 2, 2**n) tensor that receives the
+    basis (row 0 a copy of v), so that the basis can enter a GEMM."""
     ham = op.hamiltonian
     n = native.nqubits_of(v)
     if n != ham.nqubits:
@@ -112,26 +163,36 @@ def _fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
     sched = autotune.best_schedule(n, v.dtype, v.device, params, ham.complex)
     kw = sched.kwargs()
     KrylovStats.calls += 1
+    if rows is not None:
+        if rows.dim() != 2 or rows.shape[0] < max_krylov_dim + 2 or rows.shape[1] != v.numel() \
+                or rows.dtype != v.dtype or not rows.is_contiguous():
+            raise ValueError("rows must be a contiguous (max_krylov_dim + 2, 2**n) tensor of v's dtype")
+        rows[0].copy_(v)
+        v = rows[0]
+
+    def new_vec(k: int) -> torch.Tensor:
+        return rows[k] if rows is not None else torch.empty_like(v)
 
     initial_norm = math.sqrt(native.dot(n, v)[0].item())
     basis = [v]
     scales = [1.0 / initial_norm]
     T = np.zeros((max_krylov_dim + 2, max_krylov_dim + 2), dtype=np.complex128)
 
-    def combine(coeffs: np.ndarray) -> torch.Tensor:
-        k = len(coeffs)
-        c = initial_norm * np.asarray(coeffs) * np.asarray(scales[:k])
-        out = torch.empty_like(v)
-        return native.lincomb(n, basis[:k], c, out)
+    def finish(m: int, n2: float, happy: bool, converged: bool) -> LanczosRun:
+        # the result combines m+1 vectors with the extended-matrix coefficients
+        # (krylov_exp.py:88-90), or m after a happy breakdown (:66)
+        _, _, c = native.lanczos_exp(T, m, op.sigma, n2, True)
+        c = np.asarray(c[:m] if happy else c[: m + 1]).copy()
+        return LanczosRun(basis, scales, T[:m, :m].copy(), c, initial_norm, converged, happy, m,
+                          0.0 if happy else n2)
 
     # w'_0 = op(v_0)
-    wbuf = torch.empty_like(v)
+    wbuf = new_vec(1)
     d = native.hpsi(n, params, op.sigma, v, scales[0], wbuf, dots=True, complex_coef=ham.complex,
                     **kw).cpu().numpy()
     KrylovStats.matvecs += 1
     w_scale = 1.0
     ww, vw, pw, vp = d[0], complex(d[1], d[2]), 0j, 0j
-    coeffs = None
     for j in range(max_krylov_dim):
         nrm = math.sqrt(max(ww, 0.0))  # ||op(v_j)|| before orthogonalisation (:52)
         if j > 0:
@@ -141,7 +202,7 @@ def _fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
         else:
             a, b = 0j, vw
         T[j, j] = b
-        r = torch.empty_like(v)
+        r = new_vec(j + 2)
         o = native.lanczos_matvec(n, params, op.sigma, wbuf, w_scale,
                                   basis[j - 1] if j > 0 else None,
                                   scales[j - 1] if j > 0 else 0.0, a, basis[j], scales[j], b, r,
@@ -150,8 +211,7 @@ def _fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
         n2 = math.sqrt(max(o[0], 0.0))
         T[j + 1, j] = n2
         if n2 < norm_tolerance:  # happy breakdown (:63-71)
-            _, _, c = native.lanczos_exp(T, j + 1, op.sigma, 0.0, True)
-            return KrylovExpResult(combine(c[: j + 1]), True, True, j + 1)
+            return finish(j + 1, 0.0, True, True)
         basis.append(wbuf)  # z_j, normalised by its scale 1/n2
         scales.append(1.0 / n2)
         T[j + 2, j + 1] = 1.0
@@ -160,8 +220,7 @@ def _fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
         err2 = e2 * nrm
         err = e1 if e1 < err2 else (e1 * err2 / (e1 - err2))
         if err < exp_tolerance:
-            _, _, coeffs = native.lanczos_exp(T, j + 1, op.sigma, n2, True)
-            return KrylovExpResult(combine(coeffs), True, False, j + 1)
+            return finish(j + 1, n2, False, True)
         # coefficients of iteration j+1: w'_{j+1} = r/n2, v_{j+1} = z/n2,
         # <v_j|w'_{j+1}> = sigma conj(<v_{j+1}|w'_j>) / conj(sigma) (H Hermitian)
         wbuf, w_scale = r, 1.0 / n2
@@ -169,8 +228,7 @@ def _fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
         pw = op.sigma * complex(o[3], o[4]).conjugate() / (n2 * op.sigma.conjugate())
         ww = o[5] / (n2 * n2)
         vw = complex(o[6], o[7]) / (n2 * n2)
-    _, _, coeffs = native.lanczos_exp(T, max_krylov_dim, op.sigma, T[max_krylov_dim, max_krylov_dim - 1].real, True)
-    return KrylovExpResult(combine(coeffs), False, False, max_krylov_dim)
+    return finish(max_krylov_dim, T[max_krylov_dim, max_krylov_dim - 1].real, False, False)
 
 
 def _generic_krylov(op: Callable, v: torch.Tensor, is_hermitian: bool, exp_tolerance: float,
@@ -217,7 +275,9 @@ def krylov_exp_impl(op: Callable[[torch.Tensor], torch.Tensor], v: torch.Tensor,
     if isinstance(op, ScaledHamiltonian) and is_hermitian:
         if not v.is_cuda:
             raise ValueError("the fused Krylov engine needs the state on the GPU")
-        return _fused_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim)
+        run = _fused_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim)
+        return KrylovExpResult(run.combine(), run.converged, run.happy_breakdown,
+                               run.iteration_count)
     return _generic_krylov(op, v, is_hermitian, exp_tolerance, norm_tolerance, max_krylov_dim)
 
 

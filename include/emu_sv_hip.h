@@ -162,6 +162,23 @@ int emu_sv_convert(int n, int src_dtype, const void *src, int dst_dtype,
 int emu_sv_correlation(int n, int dtype, const void *psi, double *corr_out,
                        void *workspace, hipStream_t stream);
 
+/* Parameter traces of a sum of bilinear forms, the per-qubit sums behind
+ * every Hamiltonian-parameter gradient (the DHDOmegaSparse / DHDDeltaSparse /
+ * DHDPhiSparse / DHDUSparse traces of emu_sv/time_evolution/torch.py:8-119,
+ * used by its backward at :301-366).  For count pairs of device vectors
+ * W_l, Z_l (row l at element offset l*stride) and every BIT position b:
+ *   up[b]  = sum_l sum_{x: x_b=1} conj(W_l(x)) Z_l(x ^ 2^b)   (<W|s+_b|Z>)
+ *   dn[b]  = sum_l sum_{x: x_b=0} conj(W_l(x)) Z_l(x ^ 2^b)   (<W|s-_b|Z>)
+ *   occ[b] = sum_l sum_{x: x_b=1} conj(W_l(x)) Z_l(x)         (<W|n_b|Z>)
+ *   pair[b][b'] (b < b', only if want_pairs)
+ *          = sum_l sum_{x: x_b=x_b'=1} conj(W_l(x)) Z_l(x)    (<W|n_b n_b'|Z>)
+ * out (device, complex interleaved): up[n], dn[n], occ[n], pair[n*n]
+ * (6n + 2n^2 doubles; the pair block is written only if want_pairs).
+ * Deterministic (per-workgroup partials, fixed-order sums). */
+int emu_sv_parameter_traces(int n, int dtype, int count, const void *w,
+                            const void *z, uint64_t stride, int want_pairs,
+                            double *out, void *workspace, hipStream_t stream);
+
 /* Host: E = expm(A) for an m x m complex128 matrix (row-major, interleaved),
  * scaling and squaring with a [13/13] Pade approximant (Higham 2005).
  * Replaces torch.linalg.matrix_exp on the (j+3)x(j+3) extended Krylov

@@ -131,6 +131,25 @@ def dense_rydberg_hamiltonian(omegas, deltas, phis, interaction_matrix) -> torch
     return h
 
 
+def dense_rydberg_hamiltonian_autograd(omegas, deltas, phis, interaction_matrix) -> torch.Tensor:
+    """The same Kronecker-product H as dense_rydberg_hamiltonian, built from
+    the parameter tensors without detaching them, so torch autograd through
+    a dense matrix_exp gives exact parameter gradients at small N (the
+    reference's check of its backward, test_time_evolution.py:79-103)."""
+    n = len(omegas)
+    sx = torch.tensor([[0.0, 1.0], [1.0, 0.0]], dtype=DTYPE)
+    sy = torch.tensor([[0.0, -1j], [1j, 0.0]], dtype=DTYPE)
+    nop = torch.tensor([[0.0, 0.0], [0.0, 1.0]], dtype=DTYPE)
+    h = torch.zeros(2**n, 2**n, dtype=DTYPE)
+    for i in range(n):
+        h = h + omegas[i] * torch.cos(phis[i]) * _single_gate(i, n, sx) / 2
+        h = h + omegas[i] * torch.sin(phis[i]) * _single_gate(i, n, sy) / 2
+        h = h - deltas[i] * _single_gate(i, n, nop)
+        for j in range(i + 1, n):
+            h = h + interaction_matrix[i, j] * (_single_gate(i, n, nop) @ _single_gate(j, n, nop))
+    return h
+
+
 def dense_step(dt, omegas, deltas, phis, interaction_matrix, state) -> torch.Tensor:
     """expm(-i dt H) psi, as test/emu_sv/test_time_evolution.py:13-24."""
     h = dense_rydberg_hamiltonian(omegas, deltas, phis, interaction_matrix)
@@ -292,30 +311,19 @@ def step_forward(dt, omegas, deltas, phis, interaction_matrix, state, tol, max_k
     return res, ham
 
 
-def step_backward(dt, omegas, deltas, phis, interaction_matrix, state, grad_state_out,
-                  tol, max_krylov_dim):
-    """Restates DifferentiableStateVectorEvolution.backward (torch.py:221-378).
-
-    Returns (g_omegas, g_deltas, g_phis, g_interaction, g_state_in) in torch's
-    complex-gradient convention (the same convention the reference's backward
-    receives from autograd).
-    """
+def parameter_traces(bra: torch.Tensor, ket: torch.Tensor, omegas, phis, fac: complex):
+    """Re(fac * Tr(bra^dag dH/dtheta ket)) for every qubit parameter: the
+    DHDOmegaSparse / DHDDeltaSparse / DHDPhiSparse / DHDUSparse traces of
+    emu_sv/time_evolution/torch.py:8-119 as used at :318-351.  bra, ket:
+    (rows, 2**n).  Returns (g_omegas, g_deltas, g_phis, g_U upper)."""
     n = len(omegas)
-    ham = OracleHamiltonian(omegas, deltas, phis, interaction_matrix)
-
-    def op(x):
-        return -1j * dt * (ham * x)
-
-    vs, ds, vg = double_krylov(op, state.detach(), grad_state_out.detach(), tol, max_krylov_dim)
-    Vs = torch.stack(vs)
-    Vg = torch.stack(vg)
-    e_l = ds.mT @ Vs  # torch.py:318
     om = torch.as_tensor(omegas).detach()
     ph = torch.as_tensor(phis).detach()
 
     def trace(v):
-        return (-1j * dt * torch.tensordot(Vg.conj(), v)).real
+        return (fac * torch.tensordot(bra.conj(), v)).real
 
+    e_l = ket
     g_om = torch.zeros(n, dtype=torch.float64)
     g_de = torch.zeros(n, dtype=torch.float64)
     g_ph = torch.zeros(n, dtype=torch.float64)
@@ -334,6 +342,28 @@ def step_backward(dt, omegas, deltas, phis, interaction_matrix, state, grad_stat
             u[:, :, 0] = 0.0
             u[:, :, 1, :, 0] = 0.0
             g_u[i, j] = trace(u.view(e_l.shape[0], -1))
+    return g_om, g_de, g_ph, g_u
+
+
+def step_backward(dt, omegas, deltas, phis, interaction_matrix, state, grad_state_out,
+                  tol, max_krylov_dim):
+    """Restates DifferentiableStateVectorEvolution.backward (torch.py:221-378).
+
+    Returns (g_omegas, g_deltas, g_phis, g_interaction, g_state_in) in torch's
+    complex-gradient convention (the same convention the reference's backward
+    receives from autograd).
+    """
+    n = len(omegas)
+    ham = OracleHamiltonian(omegas, deltas, phis, interaction_matrix)
+
+    def op(x):
+        return -1j * dt * (ham * x)
+
+    vs, ds, vg = double_krylov(op, state.detach(), grad_state_out.detach(), tol, max_krylov_dim)
+    Vs = torch.stack(vs)
+    Vg = torch.stack(vg)
+    e_l = ds.mT @ Vs  # torch.py:318
+    g_om, g_de, g_ph, g_u = parameter_traces(Vg, e_l, omegas, phis, -1j * dt)
     g_state = krylov_exp(lambda x: (1j * dt) * (ham * x), grad_state_out.detach().clone(),
                          tol, tol, max_krylov_dim=max_krylov_dim)  # torch.py:355-366
     return g_om, g_de, g_ph, g_u, g_state
