@@ -108,7 +108,8 @@ def time_matvec(seq, dtype, reps, step):
     ms = start.elapsed_time(stop) / reps
     bpe = 16 if dtype == torch.complex128 else 8
     alg_bytes = 3 * (2**n) * bpe  # read v_j, read v_{j-1}, write v_{j+1}
-    passes = native.plan_passes(n, dtype, sched.tile_log, sched.min_contig, kernel=sched.kernel)
+    passes = native.plan_passes(n, dtype, sched.tile_log, sched.min_contig, kernel=sched.kernel,
+                                partners=sched.partners, first_log=sched.first_log)
     del w, pa, pb, r
     return ms, alg_bytes, passes, sched
 
@@ -124,7 +125,7 @@ def measured_traffic(tag, sched, n):
     except (OSError, ValueError):
         return None
     ent = rec.get(tag)
-    if not ent or rec.get("n_qubits") != n or ent.get("schedule") != sched.kwargs():
+    if not ent or rec.get("n_qubits") != n or {"partners": 0, "first_log": 0, **ent.get("schedule", {})} != sched.kwargs():
         return None
     return float(ent["bytes_per_call"])
 

@@ -26,7 +26,18 @@ EMU_SV_FLAG_COMPLEX = 1
 EMU_SV_KERNEL_DEFAULT = 0
 EMU_SV_KERNEL_LDS = 1 << 8
 EMU_SV_KERNEL_WAVE = 2 << 8
-KERNELS = {"default": EMU_SV_KERNEL_DEFAULT, "lds": EMU_SV_KERNEL_LDS, "wave": EMU_SV_KERNEL_WAVE}
+EMU_SV_KERNEL_TILE = 3 << 8
+KERNELS = {"default": EMU_SV_KERNEL_DEFAULT, "lds": EMU_SV_KERNEL_LDS, "wave": EMU_SV_KERNEL_WAVE,
+           "tile": EMU_SV_KERNEL_TILE}
+
+
+def kernel_flags(kernel: str, partners: int = 0, first_log: int = 0) -> int:
+    """Kernel-kind bits of the flags word.  Tile kernel only: ``partners`` is
+    the number of top bits the last pass may read as flip partners and
+    ``first_log`` the tile log of the first (contiguous) pass (0: tile_log)."""
+    if not 0 <= partners <= 15 or not 0 <= first_log <= 15:
+        raise ValueError("partners and first_log must be in [0, 15]")
+    return KERNELS[kernel] | (partners << 12) | (first_log << 16)
 EMU_SV_MAX_QUBITS = 32
 EMU_SV_HAM_DOUBLES = 1128
 
@@ -142,8 +153,9 @@ def pack_hamiltonian(omega, delta, phi, interaction) -> tuple[np.ndarray, bool]:
 
 
 def plan_passes(n: int, dtype: torch.dtype, tile_log: int = 0, min_contig: int = 0,
-                kernel: str = "default") -> int:
-    rc = LIB.emu_sv_plan_passes(n, dtype_code(dtype), tile_log, min_contig, KERNELS[kernel])
+                kernel: str = "default", partners: int = 0, first_log: int = 0) -> int:
+    rc = LIB.emu_sv_plan_passes(n, dtype_code(dtype), tile_log, min_contig,
+                                kernel_flags(kernel, partners, first_log))
     if rc < 0:
         check(rc, "plan_passes")
     return rc
@@ -174,7 +186,8 @@ class Workspace:
 def hpsi(n: int, ham_dev: torch.Tensor, sigma: complex, v: torch.Tensor, v_scale: float,
          w: torch.Tensor, prev: torch.Tensor | None = None, prev_scale: float = 1.0,
          dots: bool = False, complex_coef: bool = True, tile_log: int = 0,
-         min_contig: int = 0, kernel: str = "default") -> torch.Tensor | None:
+         min_contig: int = 0, kernel: str = "default", partners: int = 0,
+         first_log: int = 0) -> torch.Tensor | None:
     """w = sigma * H (v_scale v); optionally returns the 7 fused dots (device)."""
     _require_device_vec(v, "hpsi")
     _require_device_vec(w, "hpsi")
@@ -184,7 +197,7 @@ def hpsi(n: int, ham_dev: torch.Tensor, sigma: complex, v: torch.Tensor, v_scale
         raise ValueError("hpsi: prev must match v")
     ws = Workspace.get(v.device, n)
     out = ws.results[0:7] if dots else None
-    flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | KERNELS[kernel]
+    flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags(kernel, partners, first_log)
     check(LIB.emu_sv_hpsi(n, dtype_code(v.dtype), _ptr(ham_dev), float(sigma.real),
                           float(sigma.imag), _ptr(v), float(v_scale), _ptr(prev),
                           float(prev_scale), _ptr(w), _ptr(out), _ptr(ws.scratch), tile_log,
@@ -195,7 +208,8 @@ def hpsi(n: int, ham_dev: torch.Tensor, sigma: complex, v: torch.Tensor, v_scale
 def lanczos_matvec(n: int, ham_dev: torch.Tensor, sigma: complex, w: torch.Tensor, w_scale: float,
                    pa: torch.Tensor | None, pa_scale: float, a: complex, pb: torch.Tensor,
                    pb_scale: float, b: complex, r: torch.Tensor, complex_coef: bool = True,
-                   tile_log: int = 0, min_contig: int = 0, kernel: str = "default") -> torch.Tensor:
+                   tile_log: int = 0, min_contig: int = 0, kernel: str = "default",
+                   partners: int = 0, first_log: int = 0) -> torch.Tensor:
     """z = w_scale w - a pa_scale pa - b pb_scale pb (into w); r = sigma H z.
     Returns a device slice [||z||^2, Re/Im <z|pb_s>, Re/Im <z|w_s>, ||r||^2,
     Re/Im <z|r>] (pb_s = pb_scale pb, w_s = w_scale w)."""
@@ -205,7 +219,7 @@ def lanczos_matvec(n: int, ham_dev: torch.Tensor, sigma: complex, w: torch.Tenso
             raise ValueError("lanczos_matvec: vectors must be 2**n elements of one dtype")
     ws = Workspace.get(w.device, n)
     out = ws.results[24:36]
-    flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | KERNELS[kernel]
+    flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags(kernel, partners, first_log)
     check(LIB.emu_sv_lanczos_matvec(n, dtype_code(w.dtype), _ptr(ham_dev), float(sigma.real),
                                     float(sigma.imag), _ptr(w), float(w_scale), _ptr(pa),
                                     float(pa_scale), float(a.real), float(a.imag), _ptr(pb),

@@ -5,6 +5,10 @@ A schedule is (kernel kind, tile_log, min_contig):
 * "wave": a wavefront owns 2^tile_log amplitudes (tile_log = 6 lane bits +
   3..5 register bits); flips are register or cross-lane operations.
 * "lds":  a 256-thread workgroup stages 2^tile_log amplitudes in LDS.
+* "tile": a contiguous first pass (orthogonalisation + diagonal) and later
+  passes of ``min_contig`` contiguous low bits plus high bits, the last of
+  which reads the flip partners of up to ``partners`` top bits directly
+  (16-byte packets, compile-time tiles, 2^11..2^14 elements).
 
 ``tile_log`` and ``min_contig`` fix the qubit split: pass 1 covers bits
 [0, tile_log); each later pass covers the next tile_log - c high bits plus c
@@ -14,7 +18,8 @@ The first use of a (device, n, dtype, coefficient kind) with n >= 16 times
 every candidate on one Lanczos iteration (emu_sv_lanczos_matvec: two
 untimed and three timed calls each, HIP events on the current stream) and
 keeps the fastest for the rest of the process.  Set EMU_SV_AUTOTUNE=0 to use the library default, or
-EMU_SV_SCHEDULE="c128=lds:11:2,c64=lds:11:3" to fix the schedule per type.
+EMU_SV_SCHEDULE="c128=tile:12:4:6,c64=lds:11:3" (kind:tile_log:min_contig[:partners[:first_log]])
+to fix the schedule per type.
 """
 
 from __future__ import annotations
@@ -34,21 +39,28 @@ class Schedule:
     kernel: str = "default"
     tile_log: int = 0
     min_contig: int = 0
+    partners: int = 0
+    first_log: int = 0
 
     def kwargs(self) -> dict:
-        return {"kernel": self.kernel, "tile_log": self.tile_log, "min_contig": self.min_contig}
+        return {"kernel": self.kernel, "tile_log": self.tile_log, "min_contig": self.min_contig,
+                "partners": self.partners, "first_log": self.first_log}
 
 
 DEFAULT = Schedule()
 
 CANDIDATES = {
-    torch.complex64: [Schedule("wave", 9, 2), Schedule("wave", 10, 2), Schedule("wave", 10, 3),
-                      Schedule("wave", 11, 2), Schedule("wave", 11, 3), Schedule("wave", 11, 4),
-                      Schedule("lds", 11, 3), Schedule("lds", 12, 3), Schedule("lds", 13, 3),
-                      Schedule("lds", 14, 2)],
-    torch.complex128: [Schedule("wave", 9, 2), Schedule("wave", 10, 2), Schedule("wave", 10, 3),
-                       Schedule("wave", 11, 2), Schedule("wave", 11, 3), Schedule("lds", 10, 2),
-                       Schedule("lds", 11, 2), Schedule("lds", 12, 2), Schedule("lds", 13, 2)],
+    torch.complex64: [Schedule("lds", 11, 3), Schedule("wave", 11, 3),
+                      Schedule("tile", 12, 5, 0, 11), Schedule("tile", 12, 5, 0, 12),
+                      Schedule("tile", 13, 5, 0, 11), Schedule("tile", 13, 6, 0, 12),
+                      Schedule("tile", 13, 4, 4, 13), Schedule("tile", 12, 4, 6, 12),
+                      Schedule("tile", 14, 5, 4, 13), Schedule("tile", 14, 5, 6, 12),
+                      Schedule("tile", 14, 6, 6, 12), Schedule("tile", 13, 5, 6, 12)],
+    torch.complex128: [Schedule("lds", 10, 2), Schedule("lds", 11, 2), Schedule("wave", 10, 2),
+                       Schedule("tile", 12, 4, 0, 10), Schedule("tile", 12, 4, 0, 11),
+                       Schedule("tile", 12, 5, 0, 10), Schedule("tile", 13, 5, 0, 10),
+                       Schedule("tile", 12, 4, 6, 12), Schedule("tile", 13, 4, 4, 13),
+                       Schedule("tile", 13, 4, 6, 12), Schedule("tile", 13, 5, 6, 12)],
 }
 
 _cache: dict = {}
@@ -102,8 +114,11 @@ def forced_schedule(dtype: torch.dtype) -> Schedule | None:
     for item in filter(None, (x.strip() for x in spec.split(","))):
         name, _, val = item.partition("=")
         if name.strip() == key:
-            kind, tl, cmin = val.split(":")
-            return Schedule(kind.strip(), int(tl), int(cmin))
+            fields = val.split(":")
+            kind, tl, cmin = fields[:3]
+            partners = int(fields[3]) if len(fields) > 3 else 0
+            first = int(fields[4]) if len(fields) > 4 else 0
+            return Schedule(kind.strip(), int(tl), int(cmin), partners, first)
     return None
 
 

@@ -1047,6 +1047,8 @@ __global__ __launch_bounds__(kThreads) void traces_finalize_kernel(int n, int gx
   }
 }
 
+#include "emu_sv_tile.inc"
+
 // ---------------------------------------------------------------------------
 // Planner
 // ---------------------------------------------------------------------------
@@ -1223,6 +1225,162 @@ int run_hpsi(int n, const double *ham, double sr, double si, const void *v, doub
   return EMU_SV_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Tile-kernel passes: A = contiguous [0, TL) (+ diagonal, + orthogonalisation),
+// then B passes of c + m = TL tile bits; the last B pass takes up to
+// max_partner leftover top bits as partner reads.
+// ---------------------------------------------------------------------------
+struct TilePass {
+  int tl, c, s, m;
+  uint32_t fmask;
+  int pbits;
+  bool modeb;
+};
+
+// Partner counts with a compiled kernel variant.
+bool partner_count_ok(int r) { return r == 0 || r == 2 || r == 4 || r == 6; }
+
+// A: [0, tla); B passes of tlb tile bits with >= cmin contiguous low bits;
+// the last B pass may read up to max_partner leftover top bits directly.
+// Requires n >= max(tla, tlb).
+int make_tile_plan(int n, int tla, int tlb, int cmin, int max_partner, TilePass *out) {
+  int np = 0;
+  out[np++] = {tla, tla, tla, 0, bits_mask(0, tla), 0, false};
+  int next = tla;
+  while (next < n) {
+    const int rem = n - next;
+    int m = rem < tlb - cmin ? rem : tlb - cmin;
+    if (tlb - m > next) m = tlb - next;  // keep c <= s (small leftovers)
+    const int left = rem - m;
+    if (left > 0) {
+      int r = left;
+      while (r <= max_partner && !partner_count_ok(r)) ++r;
+      if (r <= max_partner && r <= rem && rem - r >= 1 && tlb - (rem - r) <= next) {
+        const int mm = rem - r;
+        out[np++] = {tlb, tlb - mm, next, mm, bits_mask(next, next + mm), r, true};
+        break;
+      }
+    }
+    out[np++] = {tlb, tlb - m, next, m, bits_mask(next, next + m), 0, true};
+    next += m;
+  }
+  return np;
+}
+
+int tile_max_log(int dtype) { return dtype == EMU_SV_C64 ? 14 : 13; }
+
+template <typename T, int TL, int NT, bool CPLX, bool MODEB, int PR>
+int launch_tile_one(const TileArgs &a, hipStream_t st) {
+  static std::once_flag once;
+  static hipError_t attr = hipSuccess;
+  const size_t lds = sizeof(typename Vec2<T>::type) << TL;
+  if (lds > 65536)
+    std::call_once(once, [&] {
+      attr = hipFuncSetAttribute(
+          reinterpret_cast<const void *>(&tile_kernel<T, TL, NT, CPLX, MODEB, PR>),
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    });
+  if (int rc = hip_check(attr, "hipFuncSetAttribute(tile kernel LDS)")) return rc;
+  const uint64_t nt = 1ull << (a.n - TL);
+  hipLaunchKernelGGL((tile_kernel<T, TL, NT, CPLX, MODEB, PR>), dim3((unsigned)nt), dim3(NT), lds,
+                     st, a);
+  return hip_check(hipGetLastError(), "tile kernel launch");
+}
+
+template <typename T, int TL, bool CPLX>
+int launch_tile_b(const TileArgs &a, hipStream_t st) {
+  constexpr int NT = TL >= 13 ? 512 : 256;
+  switch (a.partner_bits) {
+    case 0: return launch_tile_one<T, TL, NT, CPLX, true, 0>(a, st);
+    case 2: return launch_tile_one<T, TL, NT, CPLX, true, 2>(a, st);
+    case 4: return launch_tile_one<T, TL, NT, CPLX, true, 4>(a, st);
+    case 6: return launch_tile_one<T, TL, NT, CPLX, true, 6>(a, st);
+    default: return fail(EMU_SV_EINVAL, "tile kernel: unsupported partner count");
+  }
+}
+
+template <typename T, int TL>
+int launch_tile_tl(const TileArgs &a, bool cplx, bool modeb, hipStream_t st) {
+  constexpr int NT = TL >= 13 ? 512 : 256;
+  if (modeb) return cplx ? launch_tile_b<T, TL, true>(a, st) : launch_tile_b<T, TL, false>(a, st);
+  return cplx ? launch_tile_one<T, TL, NT, true, false, 0>(a, st)
+              : launch_tile_one<T, TL, NT, false, false, 0>(a, st);
+}
+
+template <typename T>
+int launch_tile(const TileArgs &a, int tl, bool cplx, bool modeb, hipStream_t st) {
+  switch (tl) {
+    case 10: return launch_tile_tl<T, 10>(a, cplx, modeb, st);
+    case 11: return launch_tile_tl<T, 11>(a, cplx, modeb, st);
+    case 12: return launch_tile_tl<T, 12>(a, cplx, modeb, st);
+    case 13: return launch_tile_tl<T, 13>(a, cplx, modeb, st);
+    case 14:
+      if constexpr (sizeof(T) == 4) return launch_tile_tl<T, 14>(a, cplx, modeb, st);
+      break;
+    default: break;
+  }
+  return fail(EMU_SV_EINVAL, "tile kernel: tile_log out of range");
+}
+
+template <typename T>
+int run_tile(int n, const double *ham, double sr, double si, const void *v, double vsc,
+             const void *prev, double psc, void *w, double *dots, void *ws, int tla, int tlb,
+             int cmin, int max_partner, bool cplx, hipStream_t st, const OrthoIn *ortho) {
+  TilePass plan[EMU_SV_MAX_QUBITS];
+  const int np = make_tile_plan(n, tla, tlb, cmin, max_partner, plan);
+  {
+    const uint64_t ntiles = 1ull << (n - tla);
+    hipLaunchKernelGGL(diag_prep_kernel, dim3((unsigned)((ntiles + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, st, n, tla, ham, const_cast<double *>(dtab_ptr(ws, n)));
+    if (int rc = hip_check(hipGetLastError(), "diag_prep launch")) return rc;
+  }
+  for (int p = 0; p < np; ++p) {
+    const bool last = p == np - 1;
+    TileArgs a{};
+    a.v = v;  // with orthogonalisation, pass A has overwritten v with z
+    a.acc = w;
+    a.out = w;
+    a.prev = last ? prev : nullptr;
+    a.ham = ham;
+    a.dtab = dtab_ptr(ws, n);
+    a.partials = partials_ptr(ws, n);
+    a.partials2 = partials2_ptr(ws, n);
+    a.ortho = (ortho != nullptr && p == 0) ? 1 : 0;
+    a.zout = const_cast<void *>(v);
+    a.pa = ortho ? ortho->pa : nullptr;
+    a.pb = ortho ? ortho->pb : nullptr;
+    a.ca_re = ortho ? ortho->ca_re : 0.0;
+    a.ca_im = ortho ? ortho->ca_im : 0.0;
+    a.cb_re = ortho ? ortho->cb_re : 0.0;
+    a.cb_im = ortho ? ortho->cb_im : 0.0;
+    a.pb_scale = ortho ? ortho->pb_scale : 0.0;
+    a.v_scale = (ortho != nullptr && p > 0) ? 1.0 : vsc;  // later passes read z = v_scale*w already
+    a.prev_scale = psc;
+    a.sig_re = sr;
+    a.sig_im = si;
+    a.n = n;
+    a.c = plan[p].c;
+    a.s = plan[p].s;
+    a.m = plan[p].m;
+    a.flip_mask = plan[p].fmask;
+    a.partner_bits = plan[p].pbits;
+    a.dots = (last && dots != nullptr) ? 1 : 0;
+    int rc = launch_tile<T>(a, plan[p].tl, cplx, plan[p].modeb, st);
+    if (rc) return rc;
+    const int nrow = (int)(1ull << (n - plan[p].tl));
+    if (a.ortho) {
+      rc = finalize(a.partials2, nrow, kOrthoSums, ws, n, ortho->norm2_out, st);
+      if (rc) return rc;
+    }
+    if (a.dots) {
+      rc = finalize(a.partials, nrow, 7, ws, n, dots, st);
+      if (rc) return rc;
+    }
+  }
+  return EMU_SV_OK;
+}
+
 // Workspace layout (bytes): [partial rows | pass-1 diagonal table |
 // finalize stage | ticket counter].  The caller zero-fills it once; the
 // ticket is re-armed by every reduction.
@@ -1293,6 +1451,28 @@ int check_dtype(int dtype) {
   return EMU_SV_OK;
 }
 
+int tile_partners(int flags) { return (flags >> 12) & 0xF; }
+
+// First-pass tile log of the tile kernel: flags bits 16-19 (0 = tile_log).
+int tile_first_log(int flags, int tl) {
+  const int t = (flags >> 16) & 0xF;
+  return t ? t : tl;
+}
+
+// 0: valid tile schedule; 1: n below one tile (use the LDS kernel); <0: error.
+int check_tile_schedule(int n, int dtype, int &tl, int &cmin, int &tla, const char *what) {
+  if (tl <= 0) tl = dtype == EMU_SV_C64 ? 13 : 12;
+  if (cmin <= 0) cmin = dtype == EMU_SV_C64 ? 5 : 4;
+  if (tla <= 0) tla = tl;
+  const int mx = tile_max_log(dtype);
+  if (tl < 10 || tl > mx || tla < 10 || tla > mx)
+    return fail(EMU_SV_EINVAL, std::string(what) + ": tile kernel needs tile logs in [10, " +
+                                   std::to_string(mx) + "]");
+  if (cmin < 1 || cmin >= tl)
+    return fail(EMU_SV_EINVAL, std::string(what) + ": min_contig must be in [1, tile_log)");
+  return (n < tl || n < tla) ? 1 : 0;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1333,6 +1513,18 @@ int emu_sv_pack_hamiltonian(int n, const double *omega, const double *delta, con
 int emu_sv_plan_passes(int n, int dtype, int tile_log, int min_contig, int flags) {
   if (int rc = check_n(n)) return rc;
   if (int rc = check_dtype(dtype)) return rc;
+  if (kernel_kind(flags) == kKernelTile) {
+    int tla = tile_first_log(flags, 0);
+    const int rc = check_tile_schedule(n, dtype, tile_log, min_contig, tla, "plan_passes");
+    if (rc < 0) return rc;
+    if (rc == 0) {
+      TilePass tplan[EMU_SV_MAX_QUBITS];
+      return make_tile_plan(n, tla, tile_log, min_contig, tile_partners(flags), tplan);
+    }
+    tile_log = 0;
+    min_contig = 0;
+    flags = EMU_SV_KERNEL_LDS;
+  }
   default_tiles(dtype, kernel_kind(flags), tile_log, min_contig);
   Pass plan[EMU_SV_MAX_QUBITS];
   return make_plan(n, tile_log, min_contig, plan);
@@ -1353,9 +1545,29 @@ int emu_sv_hpsi(int n, int dtype, const double *ham, double sigma_re, double sig
   if (!ham || !v || !w) return fail(EMU_SV_EINVAL, "hpsi: null pointer");
   if (v == w) return fail(EMU_SV_EINVAL, "hpsi: input and output must not alias");
   if (dots_out && !workspace) return fail(EMU_SV_EINVAL, "hpsi: reductions need a workspace");
-  const int kind = kernel_kind(flags);
-  if (kind != kKernelLds && kind != kKernelWave)
+  int kind = kernel_kind(flags);
+  if (kind != kKernelLds && kind != kKernelWave && kind != kKernelTile)
     return fail(EMU_SV_EINVAL, "hpsi: unknown kernel kind in flags");
+  const bool cplx = (flags & EMU_SV_FLAG_COMPLEX) != 0;
+  int tla = kind == kKernelTile ? tile_first_log(flags, 0) : 0;
+  if (kind == kKernelTile) {
+    int rc = check_tile_schedule(n, dtype, tile_log, min_contig, tla, "hpsi");
+    if (rc > 0) {
+      kind = kKernelLds;  // fewer qubits than one tile: the LDS kernel
+      tile_log = 0;
+      min_contig = 0;
+    } else if (rc < 0) {
+      return rc;
+    } else if (dtype == EMU_SV_C128) {
+      return run_tile<double>(n, ham, sigma_re, sigma_im, v, v_scale, prev, prev_scale, w, dots_out,
+                              workspace, tla, tile_log, min_contig, tile_partners(flags), cplx,
+                              stream, nullptr);
+    } else {
+      return run_tile<float>(n, ham, sigma_re, sigma_im, v, v_scale, prev, prev_scale, w, dots_out,
+                             workspace, tla, tile_log, min_contig, tile_partners(flags), cplx,
+                             stream, nullptr);
+    }
+  }
   default_tiles(dtype, kind, tile_log, min_contig);
   if (kind == kKernelWave && (tile_log < 9 || tile_log > 11))
     return fail(EMU_SV_EINVAL, "hpsi: wave tiles need tile_log in [9, 11]");
@@ -1365,7 +1577,6 @@ int emu_sv_hpsi(int n, int dtype, const double *ham, double sigma_re, double sig
     return fail(EMU_SV_EINVAL, "hpsi: complex128 tiles are limited to 2^13 elements");
   if (min_contig < 0 || min_contig >= tile_log)
     return fail(EMU_SV_EINVAL, "hpsi: min_contig must be in [0, tile_log)");
-  const bool cplx = (flags & EMU_SV_FLAG_COMPLEX) != 0;
   if (dtype == EMU_SV_C128)
     return run_hpsi<double>(n, ham, sigma_re, sigma_im, v, v_scale, prev, prev_scale, w, dots_out,
                             workspace, kind, tile_log, min_contig, cplx, stream);
@@ -1384,14 +1595,28 @@ int emu_sv_lanczos_matvec(int n, int dtype, const double *ham, double sigma_re, 
     return fail(EMU_SV_EINVAL, "lanczos_matvec: null pointer");
   if (w == r || w == pa || w == pb || r == pa || r == pb)
     return fail(EMU_SV_EINVAL, "lanczos_matvec: w, r, pa and pb must be distinct buffers");
-  const int kind = kernel_kind(flags);
-  if (kind != kKernelLds && kind != kKernelWave)
+  int kind = kernel_kind(flags);
+  if (kind != kKernelLds && kind != kKernelWave && kind != kKernelTile)
     return fail(EMU_SV_EINVAL, "lanczos_matvec: unknown kernel kind in flags");
-  default_tiles(dtype, kind, tile_log, min_contig);
+  bool use_tile = false;
+  int tla = kind == kKernelTile ? tile_first_log(flags, 0) : 0;
+  if (kind == kKernelTile) {
+    const int rc = check_tile_schedule(n, dtype, tile_log, min_contig, tla, "lanczos_matvec");
+    if (rc < 0) return rc;
+    if (rc > 0) {
+      kind = kKernelLds;
+      tile_log = 0;
+      min_contig = 0;
+    } else {
+      use_tile = true;
+    }
+  }
+  if (!use_tile) default_tiles(dtype, kind, tile_log, min_contig);
   if (kind == kKernelWave && (tile_log < 9 || tile_log > 11))
     return fail(EMU_SV_EINVAL, "lanczos_matvec: wave tiles need tile_log in [9, 11]");
-  if (tile_log < kThreadBits || tile_log > kMaxTileLog ||
-      (dtype == EMU_SV_C128 && tile_log > 13) || min_contig < 0 || min_contig >= tile_log)
+  if (!use_tile && (tile_log < kThreadBits || tile_log > kMaxTileLog ||
+                    (dtype == EMU_SV_C128 && tile_log > 13) || min_contig < 0 ||
+                    min_contig >= tile_log))
     return fail(EMU_SV_EINVAL, "lanczos_matvec: bad tile_log / min_contig");
   OrthoIn o;
   o.pa = pa;
@@ -1403,6 +1628,15 @@ int emu_sv_lanczos_matvec(int n, int dtype, const double *ham, double sigma_re, 
   o.pb_scale = pb_scale;
   o.norm2_out = norm2_out;
   const bool cplx = (flags & EMU_SV_FLAG_COMPLEX) != 0;
+  if (use_tile) {
+    if (dtype == EMU_SV_C128)
+      return run_tile<double>(n, ham, sigma_re, sigma_im, w, w_scale, nullptr, 0.0, r, dots_out,
+                              workspace, tla, tile_log, min_contig, tile_partners(flags), cplx,
+                              stream, &o);
+    return run_tile<float>(n, ham, sigma_re, sigma_im, w, w_scale, nullptr, 0.0, r, dots_out,
+                           workspace, tla, tile_log, min_contig, tile_partners(flags), cplx, stream,
+                           &o);
+  }
   if (dtype == EMU_SV_C128)
     return run_hpsi<double>(n, ham, sigma_re, sigma_im, w, w_scale, nullptr, 0.0, r, dots_out,
                             workspace, kind, tile_log, min_contig, cplx, stream, &o);

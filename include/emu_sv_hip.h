@@ -47,6 +47,16 @@ extern "C" {
 #define EMU_SV_KERNEL_LDS (1 << 8)     /* workgroup tiles staged in LDS     */
 #define EMU_SV_KERNEL_WAVE (2 << 8)    /* wavefront tiles in registers,     */
                                        /* cross-lane flips; tile_log 9..11  */
+#define EMU_SV_KERNEL_TILE (3 << 8)    /* contiguous first pass + one pass  */
+                                       /* of c contiguous low bits (min_contig) */
+                                       /* and the next high bits; tile_log  */
+                                       /* 11..13 (c128), 11..14 (c64)        */
+/* Tile kernel only: the last pass reads the flip partners of up to P
+ * leftover top bits directly (orbit-ordered tiles; 2, 4 or 6 of them are
+ * compiled), P in flags bits 12-15; the first (contiguous) pass may use its
+ * own tile log t (flags bits 16-19, 0 = tile_log). */
+#define EMU_SV_TILE_PARTNERS(p) (((p) & 0xF) << 12)
+#define EMU_SV_TILE_FIRST_LOG(t) (((t) & 0xF) << 16)
 
 #define EMU_SV_MAX_QUBITS 32
 /* Packed Hamiltonian parameter block, indexed by BIT position b = N-1-q:

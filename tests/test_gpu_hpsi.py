@@ -28,7 +28,7 @@ def _params(n, with_phase, seed):
 
 
 def _hpsi_gpu(n, om, de, ph, u, psi, dtype, sigma=1.0, tile_log=0, min_contig=0,
-              kernel="default"):
+              kernel="default", partners=0, first_log=0):
     from emulators_amd import _native as native
     block, cplx = native.pack_hamiltonian(om.numpy(), de.numpy(), ph.numpy(), u.numpy())
     dev = torch.device("cuda")
@@ -36,7 +36,7 @@ def _hpsi_gpu(n, om, de, ph, u, psi, dtype, sigma=1.0, tile_log=0, min_contig=0,
     v = psi.to(dev, dtype)
     w = torch.empty_like(v)
     native.hpsi(n, params, sigma, v, 1.0, w, complex_coef=cplx, tile_log=tile_log,
-                min_contig=min_contig, kernel=kernel)
+                min_contig=min_contig, kernel=kernel, partners=partners, first_log=first_log)
     return w.to("cpu", torch.complex128)
 
 
@@ -103,6 +103,66 @@ def test_hpsi_multipass_schedules(gpu_device, n, tile_log, min_contig, dtype):
     assert err < tol, (err, passes)
 
 
+TILE_SHAPES = [
+    # (n, tile_log, min_contig, partners, first_log, expected passes)
+    (9, 11, 4, 0, 0, None),     # fewer qubits than one tile: LDS kernel
+    (11, 11, 4, 0, 0, 1),       # one contiguous pass
+    (12, 11, 4, 2, 0, 2),       # A + B with 1 tile bit
+    (14, 11, 4, 6, 10, 2),      # A (2^10) + B (2^11 tiles)
+    (16, 11, 3, 6, 0, 2),       # A + B (5 tile bits)
+    (19, 11, 4, 2, 0, 2),       # A + B (6 tile bits) + 2 partner bits
+    (20, 11, 4, 4, 10, 2),      # A (2^10) + B (6) + 4 partner bits
+    (22, 10, 4, 6, 10, 2),      # A + B (6) + 6 partner bits
+    (21, 12, 3, 0, 0, 2),       # A + B (9 tile bits), no partners
+    (23, 11, 4, 0, 0, 3),       # A + B + B, no partners
+    (22, 13, 4, 4, 11, 2),      # 2^13 B tiles (512 threads) after a 2^11 first pass
+    (23, 12, 5, 3, 12, 3),      # A + B + B (3 leftover bits exceed no partner count <= 3)
+]
+
+
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+@pytest.mark.parametrize("n,tile_log,min_contig,partners,first_log,npass", TILE_SHAPES)
+@pytest.mark.parametrize("with_phase", [False, True])
+def test_hpsi_tile_kernel_schedules(gpu_device, n, tile_log, min_contig, partners, first_log, npass,
+                                    dtype, with_phase):
+    """The tile kernel (contiguous first pass, B passes, partner reads of the
+    top bits, 16-byte packets) over every planner shape."""
+    from emulators_amd import _native as native
+    if npass is not None:
+        assert native.plan_passes(n, dtype, tile_log, min_contig, kernel="tile",
+                                  partners=partners, first_log=first_log) == npass
+    om, de, ph, u = _params(n, with_phase, 5000 + n)
+    psi = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(19 + n))
+    if dtype == torch.complex64:
+        psi = psi.to(torch.complex64).to(torch.complex128)
+    sigma = 0.3 - 0.7j
+    ref = sigma * (OracleHamiltonian(om, de, ph, u) * psi)
+    got = _hpsi_gpu(n, om, de, ph, u, psi, dtype, sigma=sigma, tile_log=tile_log,
+                    min_contig=min_contig, kernel="tile", partners=partners, first_log=first_log)
+    tol = 1e-12 if dtype == torch.complex128 else 2e-6
+    err = float((got - ref).norm() / ref.norm())
+    assert err < tol, err
+
+
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_hpsi_tile_c64_largest_tile(gpu_device, dtype):
+    """2^14-element complex64 tiles (the largest LDS tile); complex128 stops at 2^13."""
+    from emulators_amd import _native as native
+    n = 18
+    tl = 14 if dtype == torch.complex64 else 13
+    om, de, ph, u = _params(n, True, 5100)
+    psi = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(5))
+    if dtype == torch.complex64:
+        psi = psi.to(torch.complex64).to(torch.complex128)
+    ref = OracleHamiltonian(om, de, ph, u) * psi
+    got = _hpsi_gpu(n, om, de, ph, u, psi, dtype, tile_log=tl, min_contig=5, kernel="tile",
+                    partners=2)
+    tol = 1e-12 if dtype == torch.complex128 else 2e-6
+    assert float((got - ref).norm() / ref.norm()) < tol
+    with pytest.raises(ValueError):
+        native.plan_passes(n, torch.complex128, 14, 5, kernel="tile")
+
+
 def test_hpsi_dense_and_sigma(gpu_device):
     """sigma*H against the dense Kronecker H (test_hamiltonian.py:20-85 style)."""
     n = 6
@@ -115,7 +175,7 @@ def test_hpsi_dense_and_sigma(gpu_device):
     assert torch.allclose(got, ref, atol=1e-12, rtol=0)
 
 
-@pytest.mark.parametrize("kernel", ["lds", "wave"])
+@pytest.mark.parametrize("kernel", ["lds", "wave", "tile"])
 def test_fused_dots(gpu_device, kernel):
     from emulators_amd import _native as native
     n = 14
@@ -199,7 +259,8 @@ def test_correlation(gpu_device, n):
 
 
 @pytest.mark.parametrize("kernel,n", [("lds", 7), ("lds", 13), ("wave", 12), ("wave", 17),
-                                      ("default", 20)])
+                                      ("default", 20), ("tile", 9), ("tile", 12), ("tile", 16),
+                                      ("tile", 21)])
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
 def test_lanczos_matvec(gpu_device, kernel, n, dtype):
     """z = ws*w - a*sa*pa - b*sb*pb stored into w, r = sigma H z, ||z||^2 and
@@ -214,8 +275,9 @@ def test_lanczos_matvec(gpu_device, kernel, n, dtype):
     ws_, sa, sb, a, b, sigma = 0.8, 1.7, 0.6, 0.2 - 0.3j, -0.4 + 0.1j, -0.05j
     wd, pad, pbd = w.to(dev), pa.to(dev), pb.to(dev)
     r = torch.empty_like(wd)
+    extra = {"tile_log": 11, "min_contig": 4, "partners": 4, "first_log": 10} if kernel == "tile" else {}
     o = native.lanczos_matvec(n, params, sigma, wd, ws_, pad, sa, a, pbd, sb, b, r,
-                              complex_coef=cplx, kernel=kernel).cpu().numpy()
+                              complex_coef=cplx, kernel=kernel, **extra).cpu().numpy()
     w64, pa64, pb64 = (x.to(torch.complex128) for x in (w, pa, pb))
     z = ws_ * w64 - a * sa * pa64 - b * sb * pb64
     tol = 1e-12 if dtype == torch.complex128 else 3e-6
