@@ -243,6 +243,70 @@ def adjoint_cost(reps: int = 2) -> dict:
     return out
 
 
+def batch_throughput(sample_every: int = 16) -> dict:
+    """The batching workload: 256 seeded registers, N ~ U{8..20} at random
+    positions (min spacing 5 um), Omega ~ U(1, 4 pi) rad/us, detuning ramp
+    -U(5,20) -> +U(5,20) rad/us, 500..2000 ns at dt = 10 ns, complex64 with
+    FP64 Lanczos scalars.  All 256 run in the device-driven batch engine
+    (graph-replayed batched iterations); every ``sample_every``-th register
+    also runs alone through the single-register engine (complex64), which
+    gives the sequential instances/s and a fidelity check."""
+    import torch
+
+    from emulators_amd import workloads
+    from emulators_amd.emu_base.math.krylov_exp import ScaledHamiltonian, krylov_exp
+    from emulators_amd.emu_sv import RydbergHamiltonian
+    from emulators_amd.emu_sv.batch import evolve_batch, registers_from_sequences
+
+    inst = workloads.batch_instances()
+    regs = registers_from_sequences([workloads.batch_sequence(x) for x in inst])
+    log(f"batch: {len(regs)} registers, N {min(r.nqubits for r in regs)}..{max(r.nqubits for r in regs)}, "
+        f"{sum(r.steps for r in regs)} steps in total")
+    warm = evolve_batch(regs[:8], dtype=torch.complex64)  # compile / warm the graph path
+    del warm
+    res = evolve_batch(regs, dtype=torch.complex64, check_every=64)
+    log(f"batch: {len(regs)} registers in {res.seconds:.2f} s ({res.iterations} batched iterations), "
+        f"failed {len(res.failed)}")
+    sample = list(range(0, len(regs), sample_every))
+    t_seq = 0.0
+    worst = 0.0
+    for i in sample:
+        reg = regs[i]
+        n = reg.nqubits
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        psi = torch.zeros(2**n, dtype=torch.complex64, device="cuda")
+        psi[0] = 1.0
+        for k in range(reg.steps):
+            ham = RydbergHamiltonian(torch.as_tensor(reg.omegas[k]), torch.as_tensor(reg.deltas[k]),
+                                     torch.as_tensor(reg.phis[k]),
+                                     torch.as_tensor(reg.interaction_matrix), device="cuda")
+            psi = krylov_exp(ScaledHamiltonian(ham, -1j * float(reg.dts[k])), psi, 1e-10, 1e-10)
+        torch.cuda.synchronize()
+        t_seq += time.perf_counter() - t0
+        a = psi.to(torch.complex128)
+        b = res.states[i].to(torch.complex128)
+        fid = float(torch.abs(torch.vdot(a, b)) ** 2 / (torch.vdot(a, a).real * torch.vdot(b, b).real))
+        worst = max(worst, 1.0 - fid)
+    seq_rate = len(sample) / t_seq
+    out = {
+        "workload": "256 registers, N ~ U{8..20}, random positions (min 5 um), Omega ~ U(1, 4pi), "
+                    "delta -U(5,20) -> +U(5,20), 500-2000 ns at dt=10 ns, complex64 + FP64 scalars",
+        "instances": len(regs),
+        "batched_s": res.seconds,
+        "batched_instances_per_s": len(regs) / res.seconds,
+        "batched_iterations": res.iterations,
+        "sequential_sample": len(sample),
+        "sequential_instances_per_s": seq_rate,
+        "speedup": (len(regs) / res.seconds) / seq_rate,
+        "max_fidelity_deficit_vs_sequential": worst,
+        "failed": len(res.failed),
+    }
+    log(f"batch: {out['batched_instances_per_s']:.1f} instances/s batched vs "
+        f"{seq_rate:.2f} sequential (x{out['speedup']:.1f}), 1-F <= {worst:.1e}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,6 +318,8 @@ def main() -> int:
     ap.add_argument("--no-c64", action="store_true")
     ap.add_argument("--duration", type=int, default=3000, help="pulse duration in ns")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--only-batch", action="store_true", help="time only the batching workload")
     ap.add_argument("--only-adjoint", action="store_true", help="time only the gradient workload")
     ap.add_argument("--only-matvec", action="store_true",
                     help="time only the Lanczos matvec (for profiler runs)")
@@ -278,6 +344,9 @@ def main() -> int:
     n = seq.omega.shape[1]
     log(f"workload triangular26: N={n}, {nsteps} steps; timing {steps} after {args.warmup} warmup")
 
+    if args.only_batch:
+        print(json.dumps(batch_throughput()), flush=True)
+        return 0
     if args.only_adjoint:
         print(json.dumps(adjoint_cost()), flush=True)
         return 0
@@ -327,6 +396,7 @@ def main() -> int:
     log(f"Lanczos matvec c128 {ms128:.3f} ms ({gbs128:.0f} GB/s algorithmic)")
 
     adjoint = None if args.no_adjoint else adjoint_cost()
+    batch = None if args.no_batch else batch_throughput()
     mean_iters = mv128 / steps
     cpu = None
     if not args.no_cpu_baseline:
@@ -371,6 +441,7 @@ def main() -> int:
         "c64": result64,
         "hpsi": hpsi,
         "adjoint": adjoint,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
     return 0

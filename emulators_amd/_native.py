@@ -62,6 +62,12 @@ SIGNATURES = {
     "emu_sv_convert": (_i, [_i, _i, _vp, _i, _vp, _vp]),
     "emu_sv_correlation": (_i, [_i, _i, _vp, _vp, _vp, _vp]),
     "emu_sv_parameter_traces": (_i, [_i, _i, _i, _vp, _vp, ctypes.c_uint64, _i, _vp, _vp, _vp]),
+    "emu_sv_batch_workspace_bytes": (ctypes.c_size_t, [_i, _i, _vp, _i, _i]),
+    "emu_sv_batch_create": (_i, [_i, _i, _vp, _i, _d, _i, _i, _vp, ctypes.c_size_t, _vp, _vp]),
+    "emu_sv_batch_run": (_i, [_vp, _i, _i, _vp, _vp]),
+    "emu_sv_batch_status": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "emu_sv_batch_destroy": (_i, [_vp]),
+    "emu_sv_pack_hamiltonian_steps": (_i, [_i, _i, _dp, _dp, _dp, _dp, _dp]),
     "emu_sv_expm": (_i, [_i, _dp, _dp]),
     "emu_sv_lanczos_exp": (_i, [_i, _dp, _dp, _d, _d, _d, _i, _dp]),
 }

@@ -172,8 +172,11 @@ __device__ __forceinline__ typename Vec2<T>::type load_input(const PassArgs &a, 
 // H.psi pass kernel.  CPLX: complex flip coefficients (some phi != 0).
 // DOTS: fuse the Lanczos partial sums (last pass).
 // ---------------------------------------------------------------------------
+// One tile of a pass: tile index g, diagonal constants row[0] = E_out,
+// row[1 + p] = a_p (when a.diag), partial sums into row prow of the partials.
 template <typename T, bool CPLX, bool DOTS>
-__global__ __launch_bounds__(kThreads) void hpsi_pass_kernel(const PassArgs a) {
+__device__ __forceinline__ void pass_tile(const PassArgs &a, uint64_t g, const double *row,
+                                          uint64_t prow) {
   using T2 = typename Vec2<T>::type;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T2 *tile = reinterpret_cast<T2 *>(smem_raw);
@@ -187,9 +190,6 @@ __global__ __launch_bounds__(kThreads) void hpsi_pass_kernel(const PassArgs a) {
   const int c = a.c, s = a.s, m = a.m, n = a.n;
   const int t = c + m;
   const uint32_t tsize = 1u << t;
-  const uint64_t ntiles = 1ull << (n - t);
-  uint64_t g = blockIdx.x;
-  if (a.xcd && ntiles >= 8) g = (g & 7) * (ntiles >> 3) + (g >> 3);
   const int nlo = s - c;
   const uint64_t base = ((g & ((1ull << nlo) - 1)) << c) | ((g >> nlo) << (s + m));
   const uint32_t cmask = (1u << c) - 1;
@@ -204,8 +204,7 @@ __global__ __launch_bounds__(kThreads) void hpsi_pass_kernel(const PassArgs a) {
   double osum[kOrthoSums] = {0, 0, 0, 0, 0};
   for (uint32_t l = tid; l < tsize; l += kThreads) tile[l] = load_input<T>(a, gidx(l), osum);
   if (a.diag) {
-    // Outer-bit constants of this tile, tabulated by diag_prep_kernel.
-    const double *row = a.dtab + g * kDiagRow;
+    // Outer-bit constants of this tile.
     if (tid < t) s_ap[tid] = row[1 + tid];
     if (tid == 63) s_eout = row[0];
     for (int e = tid; e < t * t; e += kThreads) {
@@ -325,11 +324,20 @@ __global__ __launch_bounds__(kThreads) void hpsi_pass_kernel(const PassArgs a) {
       }
     }
   }
-  if (DOTS) block_reduce_store<7>(dots, s_red, a.partials + (uint64_t)blockIdx.x * kNumDots);
+  if (DOTS) block_reduce_store<7>(dots, s_red, a.partials + prow * kNumDots);
   if (a.ortho) {
     __syncthreads();
-    block_reduce_store<kOrthoSums>(osum, s_red, a.partials2 + (uint64_t)blockIdx.x * kNumDots);
+    block_reduce_store<kOrthoSums>(osum, s_red, a.partials2 + prow * kNumDots);
   }
+}
+
+template <typename T, bool CPLX, bool DOTS>
+__global__ __launch_bounds__(kThreads) void hpsi_pass_kernel(const PassArgs a) {
+  const int t = a.c + a.m;
+  const uint64_t ntiles = 1ull << (a.n - t);
+  uint64_t g = blockIdx.x;
+  if (a.xcd && ntiles >= 8) g = (g & 7) * (ntiles >> 3) + (g >> 3);
+  pass_tile<T, CPLX, DOTS>(a, g, a.diag ? a.dtab + g * kDiagRow : nullptr, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1048,6 +1056,7 @@ __global__ __launch_bounds__(kThreads) void traces_finalize_kernel(int n, int gx
 }
 
 #include "emu_sv_tile.inc"
+#include "emu_sv_batch.inc"
 
 // ---------------------------------------------------------------------------
 // Planner
@@ -1379,6 +1388,107 @@ int run_tile(int n, const double *ham, double sr, double si, const void *v, doub
     }
   }
   return EMU_SV_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Batch engine host side.
+// ---------------------------------------------------------------------------
+struct BatchHandle {
+  int count = 0, dtype = 0, max_dim = 0, npass_max = 0;
+  double tol = 0.0;
+  BatchReg *regs = nullptr;
+  BatchEntry *tab[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint64_t ntab[4] = {0, 0, 0, 0};
+  BatchEntry *comb = nullptr;
+  uint64_t ncomb = 0;
+  int *counter = nullptr;
+  size_t lds = 0;
+  hipStream_t capture_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<BatchReg> host_regs;
+};
+
+struct BatchLayout {
+  size_t regs, arrays, partials, tables, counter, total;
+  uint64_t ntab[4], ncomb;
+};
+
+int batch_plan(const emu_sv_batch_register &r, int dtype, int tl, int cmin, BatchReg &out) {
+  if (r.n < 1 || r.n > 30 || r.steps < 1 || !r.ham || !r.dt || !r.psi || !r.psi_alt || !r.rows)
+    return fail(EMU_SV_EINVAL, "batch: bad register description");
+  const int t = r.n < tl ? r.n : tl;
+  Pass plan[EMU_SV_MAX_QUBITS];
+  const int np = make_plan(r.n, t, cmin, plan);
+  if (np > 4) return fail(EMU_SV_EINVAL, "batch: register needs more than 4 passes");
+  out = BatchReg{};
+  out.n = r.n;
+  out.steps = r.steps;
+  out.cplx = (r.flags & EMU_SV_FLAG_COMPLEX) ? 1 : 0;
+  out.npass = np;
+  out.ntiles_max = 0;
+  for (int p = 0; p < np; ++p) {
+    out.plan[p] = {plan[p].c, plan[p].s, plan[p].m, plan[p].diag, plan[p].fmask};
+    out.tiles[p] = 1ull << (r.n - (plan[p].c + plan[p].m));
+    if (out.tiles[p] > out.ntiles_max) out.ntiles_max = out.tiles[p];
+  }
+  (void)dtype;
+  return EMU_SV_OK;
+}
+
+int batch_layout(int count, const emu_sv_batch_register *regs, int dtype, int tl, int cmin,
+                 BatchLayout &L, std::vector<BatchReg> *built) {
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  L = BatchLayout{};
+  size_t off = 0;
+  L.regs = off;
+  off = al(off + sizeof(BatchReg) * (size_t)count);
+  L.arrays = off;
+  off = al(off + sizeof(double) * BatchArr::kTotal * (size_t)count);
+  L.partials = off;
+  uint64_t part = 0;
+  for (int i = 0; i < count; ++i) {
+    BatchReg br;
+    if (int rc = batch_plan(regs[i], dtype, tl, cmin, br)) return rc;
+    part += 2 * br.ntiles_max * kNumDots;
+    for (int p = 0; p < br.npass; ++p) L.ntab[p] += br.tiles[p];
+    L.ncomb += ((1ull << br.n) + kBatchChunk - 1) / kBatchChunk;
+    if (built) built->push_back(br);
+  }
+  off = al(off + sizeof(double) * part);
+  L.tables = off;
+  uint64_t ent = L.ncomb;
+  for (int p = 0; p < 4; ++p) ent += L.ntab[p];
+  off = al(off + sizeof(BatchEntry) * ent);
+  L.counter = off;
+  off = al(off + 256);
+  L.total = off;
+  return EMU_SV_OK;
+}
+
+int batch_tile_logs(int dtype, int &tl, int &cmin) {
+  if (tl <= 0) tl = dtype == EMU_SV_C64 ? 11 : 10;
+  if (cmin <= 0) cmin = dtype == EMU_SV_C64 ? 3 : 2;
+  if (tl < kThreadBits || tl > 12 || cmin < 1 || cmin >= tl)
+    return fail(EMU_SV_EINVAL, "batch: tile_log must be in [8, 12], min_contig in [1, tile_log)");
+  return EMU_SV_OK;
+}
+
+template <typename T>
+int batch_enqueue_iteration(BatchHandle &h, hipStream_t st) {
+  for (int p = 0; p < h.npass_max; ++p) {
+    if (h.ntab[p] == 0) continue;
+    hipLaunchKernelGGL(batch_pass_kernel<T>, dim3((unsigned)h.ntab[p]), dim3(kThreads), h.lds, st,
+                       h.regs, h.tab[p], p);
+    if (int rc = hip_check(hipGetLastError(), "batch pass launch")) return rc;
+  }
+  hipLaunchKernelGGL(batch_decide_kernel, dim3((unsigned)h.count), dim3(64), 0, st, h.regs, h.tol,
+                     h.max_dim);
+  if (int rc = hip_check(hipGetLastError(), "batch decide launch")) return rc;
+  hipLaunchKernelGGL(batch_combine_kernel<T>, dim3((unsigned)h.ncomb), dim3(kThreads), 0, st, h.regs,
+                     h.comb);
+  return hip_check(hipGetLastError(), "batch combine launch");
 }
 
 // Workspace layout (bytes): [partial rows | pass-1 diagonal table |
@@ -1810,6 +1920,206 @@ int emu_sv_parameter_traces(int n, int dtype, int count, const void *w, const vo
   hipLaunchKernelGGL(traces_finalize_kernel, dim3((nout + kThreads - 1) / kThreads), dim3(kThreads), 0,
                      stream, n, (int)gx, want_pairs, (const double *)part, out);
   return hip_check(hipGetLastError(), "parameter_traces finalize launch");
+}
+
+size_t emu_sv_batch_workspace_bytes(int count, int dtype, const emu_sv_batch_register *regs,
+                                    int tile_log, int min_contig) {
+  if (count < 1 || !regs || check_dtype(dtype)) return 0;
+  if (batch_tile_logs(dtype, tile_log, min_contig)) return 0;
+  BatchLayout L;
+  if (batch_layout(count, regs, dtype, tile_log, min_contig, L, nullptr)) return 0;
+  return L.total;
+}
+
+int emu_sv_batch_create(int count, int dtype, const emu_sv_batch_register *regs, int max_dim,
+                        double tolerance, int tile_log, int min_contig, void *workspace,
+                        size_t workspace_bytes, hipStream_t stream, void **handle_out) {
+  if (!handle_out) return fail(EMU_SV_EINVAL, "batch_create: null handle pointer");
+  *handle_out = nullptr;
+  if (count < 1 || !regs || !workspace) return fail(EMU_SV_EINVAL, "batch_create: bad arguments");
+  if (int rc = check_dtype(dtype)) return rc;
+  if (max_dim < 2 || max_dim > kBatchMaxDim)
+    return fail(EMU_SV_EINVAL, "batch_create: max_dim must be in [2, 63]");
+  if (!(tolerance > 0.0)) return fail(EMU_SV_EINVAL, "batch_create: tolerance must be > 0");
+  if (int rc = batch_tile_logs(dtype, tile_log, min_contig)) return rc;
+  auto *h = new BatchHandle();
+  BatchLayout L;
+  if (int rc = batch_layout(count, regs, dtype, tile_log, min_contig, L, &h->host_regs)) {
+    delete h;
+    return rc;
+  }
+  if (workspace_bytes < L.total) {
+    delete h;
+    return fail(EMU_SV_EINVAL, "batch_create: workspace too small");
+  }
+  char *base = reinterpret_cast<char *>(workspace);
+  h->count = count;
+  h->dtype = dtype;
+  h->max_dim = max_dim;
+  h->tol = tolerance;
+  h->regs = reinterpret_cast<BatchReg *>(base + L.regs);
+  double *arrays = reinterpret_cast<double *>(base + L.arrays);
+  double *partials = reinterpret_cast<double *>(base + L.partials);
+  BatchEntry *tables = reinterpret_cast<BatchEntry *>(base + L.tables);
+  h->counter = reinterpret_cast<int *>(base + L.counter);
+  const size_t esz = dtype == EMU_SV_C64 ? sizeof(float2) : sizeof(double2);
+  // host copies of everything the device needs
+  std::vector<BatchEntry> tab_host;
+  uint64_t toff[4], coff;
+  {
+    uint64_t o = 0;
+    for (int p = 0; p < 4; ++p) {
+      toff[p] = o;
+      o += L.ntab[p];
+    }
+    coff = o;
+    tab_host.resize(o + L.ncomb);
+  }
+  uint64_t tfill[4] = {0, 0, 0, 0}, cfill = 0, poff = 0;
+  int max_t = 0;
+  for (int i = 0; i < count; ++i) {
+    BatchReg &R = h->host_regs[i];
+    const emu_sv_batch_register &U = regs[i];
+    R.ham = U.ham;
+    R.dt = U.dt;
+    R.psi[0] = U.psi;
+    R.psi[1] = U.psi_alt;
+    R.user_psi = U.psi;
+    R.rows = U.rows;
+    R.partials = partials + poff;
+    poff += 2 * R.ntiles_max * kNumDots;
+    R.arr = arrays + (size_t)i * BatchArr::kTotal;
+    R.cur = 0;
+    R.step = 0;
+    R.j = 0;
+    R.phase = kPhStart;
+    R.m = 0;
+    R.ticket = 0;
+    R.matvecs = 0;
+    double dt0 = 0.0;
+    if (int rc = hip_check(hipMemcpy(&dt0, U.dt, sizeof(double), hipMemcpyDeviceToHost),
+                           "batch_create: read dt")) {
+      delete h;
+      return rc;
+    }
+    R.sig_re = 0.0;
+    R.sig_im = -dt0;
+    for (int p = 0; p < R.npass; ++p) {
+      const int t = R.plan[p].c + R.plan[p].m;
+      if (t > max_t) max_t = t;
+      for (uint64_t g = 0; g < R.tiles[p]; ++g) tab_host[toff[p] + tfill[p]++] = {(uint32_t)i, (uint32_t)g};
+      if (p + 1 > h->npass_max) h->npass_max = p + 1;
+    }
+    const uint64_t nch = ((1ull << R.n) + kBatchChunk - 1) / kBatchChunk;
+    for (uint64_t c = 0; c < nch; ++c) tab_host[coff + cfill++] = {(uint32_t)i, (uint32_t)c};
+  }
+  for (int p = 0; p < 4; ++p) {
+    h->ntab[p] = L.ntab[p];
+    h->tab[p] = tables + toff[p];
+  }
+  h->ncomb = L.ncomb;
+  h->comb = tables + coff;
+  h->lds = esz << max_t;
+  std::vector<double> arr_zero((size_t)count * BatchArr::kTotal, 0.0);
+  int rc = hip_check(hipMemcpyAsync(h->regs, h->host_regs.data(), sizeof(BatchReg) * count,
+                                    hipMemcpyHostToDevice, stream), "batch_create: upload");
+  if (!rc) rc = hip_check(hipMemcpyAsync(arrays, arr_zero.data(), sizeof(double) * arr_zero.size(),
+                                         hipMemcpyHostToDevice, stream), "batch_create: upload");
+  if (!rc) rc = hip_check(hipMemcpyAsync(tables, tab_host.data(), sizeof(BatchEntry) * tab_host.size(),
+                                         hipMemcpyHostToDevice, stream), "batch_create: upload");
+  if (!rc) rc = hip_check(hipStreamSynchronize(stream), "batch_create: sync");
+  if (!rc) rc = hip_check(hipStreamCreateWithFlags(&h->capture_stream, hipStreamNonBlocking),
+                          "batch_create: stream");
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  *handle_out = h;
+  return EMU_SV_OK;
+}
+
+int emu_sv_batch_run(void *handle, int max_iterations, int check_every, hipStream_t stream,
+                     int *remaining_out) {
+  auto *h = reinterpret_cast<BatchHandle *>(handle);
+  if (!h || max_iterations < 0 || check_every < 1) return fail(EMU_SV_EINVAL, "batch_run: bad arguments");
+  if (!h->exec) {
+    // capture one batched iteration (passes, decide, combine) once
+    hipStream_t cs = h->capture_stream;
+    int rc = hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "batch_run: capture");
+    if (!rc) rc = h->dtype == EMU_SV_C64 ? batch_enqueue_iteration<float>(*h, cs)
+                                         : batch_enqueue_iteration<double>(*h, cs);
+    hipGraph_t g = nullptr;
+    const int rc2 = hip_check(hipStreamEndCapture(cs, &g), "batch_run: end capture");
+    if (rc || rc2) {
+      if (g) hipGraphDestroy(g);
+      return rc ? rc : rc2;
+    }
+    h->graph = g;
+    if (int rc3 = hip_check(hipGraphInstantiate(&h->exec, g, nullptr, nullptr, 0), "batch_run: instantiate"))
+      return rc3;
+  }
+  int remaining = 1;
+  for (int it = 0; it < max_iterations; ++it) {
+    if (int rc = hip_check(hipGraphLaunch(h->exec, stream), "batch_run: graph launch")) return rc;
+    if ((it + 1) % check_every == 0 || it + 1 == max_iterations) {
+      hipLaunchKernelGGL(batch_count_kernel, dim3(1), dim3(kThreads), 0, stream, h->regs, h->count,
+                         h->counter);
+      if (int rc = hip_check(hipGetLastError(), "batch count launch")) return rc;
+      if (int rc = hip_check(hipMemcpyAsync(&remaining, h->counter, sizeof(int), hipMemcpyDeviceToHost,
+                                            stream), "batch_run: read counter")) return rc;
+      if (int rc = hip_check(hipStreamSynchronize(stream), "batch_run: sync")) return rc;
+      if (remaining == 0) break;
+    }
+  }
+  if (remaining == 0) {
+    const BatchEntry *comb = h->comb;
+    if (h->dtype == EMU_SV_C64)
+      hipLaunchKernelGGL(batch_publish_kernel<float>, dim3((unsigned)h->ncomb), dim3(kThreads), 0, stream,
+                         h->regs, comb);
+    else
+      hipLaunchKernelGGL(batch_publish_kernel<double>, dim3((unsigned)h->ncomb), dim3(kThreads), 0, stream,
+                         h->regs, comb);
+    if (int rc = hip_check(hipGetLastError(), "batch publish launch")) return rc;
+  }
+  if (remaining_out) *remaining_out = remaining;
+  return EMU_SV_OK;
+}
+
+int emu_sv_batch_status(void *handle, int *phase_out, int *steps_out, int *matvecs_out,
+                        hipStream_t stream) {
+  auto *h = reinterpret_cast<BatchHandle *>(handle);
+  if (!h) return fail(EMU_SV_EINVAL, "batch_status: null handle");
+  std::vector<BatchReg> r(h->count);
+  if (int rc = hip_check(hipMemcpyAsync(r.data(), h->regs, sizeof(BatchReg) * h->count,
+                                        hipMemcpyDeviceToHost, stream), "batch_status: read")) return rc;
+  if (int rc = hip_check(hipStreamSynchronize(stream), "batch_status: sync")) return rc;
+  for (int i = 0; i < h->count; ++i) {
+    if (phase_out) phase_out[i] = r[i].phase;
+    if (steps_out) steps_out[i] = r[i].step;
+    if (matvecs_out) matvecs_out[i] = r[i].matvecs;
+  }
+  return EMU_SV_OK;
+}
+
+int emu_sv_batch_destroy(void *handle) {
+  auto *h = reinterpret_cast<BatchHandle *>(handle);
+  if (!h) return EMU_SV_OK;
+  if (h->exec) hipGraphExecDestroy(h->exec);
+  if (h->graph) hipGraphDestroy(h->graph);
+  if (h->capture_stream) hipStreamDestroy(h->capture_stream);
+  delete h;
+  return EMU_SV_OK;
+}
+
+int emu_sv_pack_hamiltonian_steps(int n, int steps, const double *omega, const double *delta,
+                                  const double *phi, const double *interaction, double *blocks_out) {
+  if (steps < 1 || !blocks_out) return fail(EMU_SV_EINVAL, "pack_hamiltonian_steps: bad arguments");
+  for (int k = 0; k < steps; ++k)
+    if (int rc = emu_sv_pack_hamiltonian(n, omega + (size_t)k * n, delta + (size_t)k * n,
+                                         phi + (size_t)k * n, interaction,
+                                         blocks_out + (size_t)k * EMU_SV_HAM_DOUBLES))
+      return rc;
+  return EMU_SV_OK;
 }
 
 }  // extern "C"

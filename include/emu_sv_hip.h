@@ -189,6 +189,58 @@ int emu_sv_parameter_traces(int n, int dtype, int count, const void *w,
                             const void *z, uint64_t stride, int want_pairs,
                             double *out, void *workspace, hipStream_t stream);
 
+/* ---- many registers in one device-driven engine ------------------------ */
+/* One register of a batch.  All pointers are device memory owned by the
+ * caller.  psi holds the initial state and receives the final state. */
+typedef struct emu_sv_batch_register {
+  int n;              /* qubits (1..30) */
+  int steps;          /* time steps */
+  int flags;          /* EMU_SV_FLAG_COMPLEX if any step has phi != 0 */
+  const double *ham;  /* steps * EMU_SV_HAM_DOUBLES packed blocks, one per step */
+  const double *dt;   /* steps time steps in us (op = -i dt H) */
+  void *psi;          /* 2^n elements: initial state in, final state out */
+  void *psi_alt;      /* 2^n elements of scratch */
+  void *rows;         /* (max_dim + 1) * 2^n elements of scratch (Krylov basis) */
+} emu_sv_batch_register;
+
+/* Bytes of device workspace emu_sv_batch_create needs for these registers. */
+size_t emu_sv_batch_workspace_bytes(int count, int dtype, const emu_sv_batch_register *regs,
+                                    int tile_log, int min_contig);
+
+/* Set up a batch: every register runs the Krylov time stepping of
+ * emu_base/math/krylov_exp.py:23-131 (the arithmetic of emu_sv_lanczos_matvec,
+ * the Expokit test of emu_sv_lanczos_exp) step after step
+ * (emu_sv/sv_backend_impl.py:186-209), with its whole state machine in device
+ * memory, so one batched iteration is a fixed kernel sequence (captured once
+ * in a hipGraph).  max_dim <= 63; tile_log 8..12 / min_contig: the H.psi pass
+ * split (0 = default).  The library allocates no device memory: the
+ * workspace (zero-filled not required) is the caller's.  *handle_out is a
+ * host object released by emu_sv_batch_destroy.  No counterpart in the
+ * reference (it evolves one register per call). */
+int emu_sv_batch_create(int count, int dtype, const emu_sv_batch_register *regs, int max_dim,
+                        double tolerance, int tile_log, int min_contig, void *workspace,
+                        size_t workspace_bytes, hipStream_t stream, void **handle_out);
+
+/* Replay the batched iteration up to max_iterations times on stream, checking
+ * every check_every iterations how many registers are still running
+ * (*remaining_out); when none is, the final states are copied to each
+ * register's psi.  Registers whose Krylov iteration did not converge within
+ * max_dim stop (phase 4 in emu_sv_batch_status). */
+int emu_sv_batch_run(void *handle, int max_iterations, int check_every, hipStream_t stream,
+                     int *remaining_out);
+
+/* Per register (host arrays of count ints, any may be NULL): phase (0 start,
+ * 1 iterating, 2 combining, 3 done, 4 failed), completed steps, matvecs. */
+int emu_sv_batch_status(void *handle, int *phase_out, int *steps_out, int *matvecs_out,
+                        hipStream_t stream);
+
+int emu_sv_batch_destroy(void *handle);
+
+/* Pack steps parameter blocks (omega/delta/phi: steps x n, row-major;
+ * interaction n x n) into blocks_out (steps * EMU_SV_HAM_DOUBLES, host). */
+int emu_sv_pack_hamiltonian_steps(int n, int steps, const double *omega, const double *delta,
+                                  const double *phi, const double *interaction, double *blocks_out);
+
 /* Host: E = expm(A) for an m x m complex128 matrix (row-major, interleaved),
  * scaling and squaring with a [13/13] Pade approximant (Higham 2005).
  * Replaces torch.linalg.matrix_exp on the (j+3)x(j+3) extended Krylov

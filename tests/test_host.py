@@ -347,3 +347,38 @@ def test_forced_schedule_env(monkeypatch):
     assert autotune.forced_schedule(torch.complex64) == autotune.Schedule("wave", 10, 3)
     monkeypatch.delenv("EMU_SV_SCHEDULE")
     assert autotune.forced_schedule(torch.complex64) is None
+
+
+def test_pack_hamiltonian_steps_matches_single_blocks():
+    import ctypes
+
+    from emulators_amd import _native as native
+
+    rng = np.random.default_rng(3)
+    n, s = 5, 4
+    om, de, ph = (np.ascontiguousarray(rng.normal(size=(s, n))) for _ in range(3))
+    u = rng.uniform(size=(n, n))
+    u = np.ascontiguousarray(u + u.T)
+    blocks = np.zeros((s, native.EMU_SV_HAM_DOUBLES))
+    dp = ctypes.POINTER(ctypes.c_double)
+    native.check(native.LIB.emu_sv_pack_hamiltonian_steps(
+        n, s, om.ctypes.data_as(dp), de.ctypes.data_as(dp), ph.ctypes.data_as(dp),
+        u.ctypes.data_as(dp), blocks.ctypes.data_as(dp)), "pack steps")
+    for k in range(s):
+        ref, _ = native.pack_hamiltonian(om[k], de[k], ph[k], u)
+        assert np.array_equal(blocks[k], ref)
+
+
+def test_batch_engine_rejects_cpu_and_bad_registers():
+    import ctypes
+
+    from emulators_amd import _native as native
+    from emulators_amd.emu_sv.batch import BatchRegister, _Reg, evolve_batch
+
+    reg = BatchRegister(np.ones((2, 3)), np.zeros((2, 3)), np.zeros((2, 3)), np.zeros((3, 3)),
+                        np.full(2, 0.01))
+    with pytest.raises(ValueError):
+        evolve_batch([reg], device="cpu")
+    bad = (_Reg * 1)()  # null pointers: the workspace query refuses it
+    assert native.LIB.emu_sv_batch_workspace_bytes(1, native.EMU_SV_C64, bad, 0, 0) == 0
+    assert native.LIB.emu_sv_batch_destroy(None) == 0
