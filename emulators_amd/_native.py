@@ -67,7 +67,12 @@ SIGNATURES = {
     "emu_sv_batch_run": (_i, [_vp, _i, _i, _vp, _vp]),
     "emu_sv_batch_status": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "emu_sv_batch_destroy": (_i, [_vp]),
+    "emu_sv_batch_resume": (_i, [_vp, _vp]),
+    "emu_sv_batch_publish": (_i, [_vp, _vp]),
+    "emu_sv_batch_step_info": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "emu_sv_batch_array_layout": (_i, [_vp]),
     "emu_sv_pack_hamiltonian_steps": (_i, [_i, _i, _dp, _dp, _dp, _dp, _dp]),
+    "emu_sv_tridiag_eig": (_i, [_i, _dp, _dp, _dp, _dp]),
     "emu_sv_expm": (_i, [_i, _dp, _dp]),
     "emu_sv_lanczos_exp": (_i, [_i, _dp, _dp, _d, _d, _d, _i, _dp]),
 }
@@ -345,6 +350,19 @@ def expm(a: np.ndarray) -> np.ndarray:
     check(LIB.emu_sv_expm(m, a.view(np.float64).ctypes.data_as(dp),
                           e.view(np.float64).ctypes.data_as(dp)), "expm")
     return e
+
+
+def tridiag_eig(d: np.ndarray, e: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """(lambda, Z) of the real symmetric tridiagonal (d, e): J = Z diag(lambda) Z^T."""
+    d = np.ascontiguousarray(np.asarray(d, dtype=np.float64))
+    m = d.shape[0]
+    e = np.ascontiguousarray(np.asarray(e, dtype=np.float64)) if m > 1 else np.zeros(1)
+    lam = np.empty(m)
+    z = np.empty((m, m))
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(LIB.emu_sv_tridiag_eig(m, d.ctypes.data_as(dp), e.ctypes.data_as(dp), lam.ctypes.data_as(dp),
+                                 z.ctypes.data_as(dp)), "tridiag_eig")
+    return lam, z
 
 
 def lanczos_exp(T: np.ndarray, m: int, sigma: complex, n2: float, want_coeffs: bool):

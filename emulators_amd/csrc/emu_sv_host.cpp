@@ -268,3 +268,15 @@ extern "C" int emu_sv_lanczos_exp(int m, const double *tdiag, const double *tsub
   }
   return EMU_SV_OK;
 }
+
+extern "C" int emu_sv_tridiag_eig(int m, const double *d_in, const double *e_in, double *lambda_out,
+                                  double *z_out) {
+  if (m < 1 || m > 4096 || !d_in || (m > 1 && !e_in) || !lambda_out || !z_out) return EMU_SV_EINVAL;
+  std::vector<double> d(d_in, d_in + m), e(m > 1 ? m - 1 : 1, 0.0);
+  for (int k = 0; k + 1 < m; ++k) e[k] = e_in[k];
+  std::vector<int> rows(m);
+  for (int k = 0; k < m; ++k) rows[k] = k;
+  if (!tridiag_eig(m, d.data(), e.data(), m, rows.data(), z_out)) return EMU_SV_EINVAL;
+  for (int k = 0; k < m; ++k) lambda_out[k] = d[k];
+  return EMU_SV_OK;
+}

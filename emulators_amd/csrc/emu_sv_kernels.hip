@@ -1427,6 +1427,7 @@ int batch_plan(const emu_sv_batch_register &r, int dtype, int tl, int cmin, Batc
   out.steps = r.steps;
   out.cplx = (r.flags & EMU_SV_FLAG_COMPLEX) ? 1 : 0;
   out.npass = np;
+  out.pause = r.pause_each_step ? 1 : 0;
   out.ntiles_max = 0;
   for (int p = 0; p < np; ++p) {
     out.plan[p] = {plan[p].c, plan[p].s, plan[p].m, plan[p].diag, plan[p].fmask};
@@ -2071,16 +2072,6 @@ int emu_sv_batch_run(void *handle, int max_iterations, int check_every, hipStrea
       if (remaining == 0) break;
     }
   }
-  if (remaining == 0) {
-    const BatchEntry *comb = h->comb;
-    if (h->dtype == EMU_SV_C64)
-      hipLaunchKernelGGL(batch_publish_kernel<float>, dim3((unsigned)h->ncomb), dim3(kThreads), 0, stream,
-                         h->regs, comb);
-    else
-      hipLaunchKernelGGL(batch_publish_kernel<double>, dim3((unsigned)h->ncomb), dim3(kThreads), 0, stream,
-                         h->regs, comb);
-    if (int rc = hip_check(hipGetLastError(), "batch publish launch")) return rc;
-  }
   if (remaining_out) *remaining_out = remaining;
   return EMU_SV_OK;
 }
@@ -2098,6 +2089,61 @@ int emu_sv_batch_status(void *handle, int *phase_out, int *steps_out, int *matve
     if (steps_out) steps_out[i] = r[i].step;
     if (matvecs_out) matvecs_out[i] = r[i].matvecs;
   }
+  return EMU_SV_OK;
+}
+
+int emu_sv_batch_publish(void *handle, hipStream_t stream) {
+  auto *h = reinterpret_cast<BatchHandle *>(handle);
+  if (!h) return fail(EMU_SV_EINVAL, "batch_publish: null handle");
+  if (h->dtype == EMU_SV_C64)
+    hipLaunchKernelGGL(batch_publish_kernel<float>, dim3((unsigned)h->ncomb), dim3(kThreads), 0, stream,
+                       h->regs, h->comb);
+  else
+    hipLaunchKernelGGL(batch_publish_kernel<double>, dim3((unsigned)h->ncomb), dim3(kThreads), 0, stream,
+                       h->regs, h->comb);
+  return hip_check(hipGetLastError(), "batch publish launch");
+}
+
+int emu_sv_batch_resume(void *handle, hipStream_t stream) {
+  auto *h = reinterpret_cast<BatchHandle *>(handle);
+  if (!h) return fail(EMU_SV_EINVAL, "batch_resume: null handle");
+  hipLaunchKernelGGL(batch_resume_kernel, dim3((h->count + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                     stream, h->regs, h->count);
+  return hip_check(hipGetLastError(), "batch_resume launch");
+}
+
+int emu_sv_batch_step_info(void *handle, int reg, int *info_out, double *scalars_out, double *arrays_out,
+                           void **state_out, void **prev_state_out, void **rows_out, hipStream_t stream) {
+  auto *h = reinterpret_cast<BatchHandle *>(handle);
+  if (!h || reg < 0 || reg >= h->count || !info_out || !scalars_out || !arrays_out)
+    return fail(EMU_SV_EINVAL, "batch_step_info: bad arguments");
+  BatchReg r;
+  if (int rc = hip_check(hipMemcpyAsync(&r, h->regs + reg, sizeof(BatchReg), hipMemcpyDeviceToHost, stream),
+                         "batch_step_info: read")) return rc;
+  if (int rc = hip_check(hipStreamSynchronize(stream), "batch_step_info: sync")) return rc;
+  if (int rc = hip_check(hipMemcpyAsync(arrays_out, r.arr, sizeof(double) * BatchArr::kTotal,
+                                        hipMemcpyDeviceToHost, stream), "batch_step_info: read")) return rc;
+  if (int rc = hip_check(hipStreamSynchronize(stream), "batch_step_info: sync")) return rc;
+  info_out[0] = r.phase;
+  info_out[1] = r.step;
+  info_out[2] = r.m;
+  info_out[3] = r.happy;
+  info_out[4] = r.matvecs;
+  scalars_out[0] = r.initial_norm;
+  if (state_out) *state_out = r.psi[r.cur];
+  if (prev_state_out) *prev_state_out = r.psi[r.cur ^ 1];
+  if (rows_out) *rows_out = r.rows;
+  return EMU_SV_OK;
+}
+
+int emu_sv_batch_array_layout(int *offsets_out) {
+  if (!offsets_out) return fail(EMU_SV_EINVAL, "batch_array_layout: null pointer");
+  offsets_out[0] = BatchArr::kDiag;
+  offsets_out[1] = BatchArr::kSub;
+  offsets_out[2] = BatchArr::kScale;
+  offsets_out[3] = BatchArr::kCoef;
+  offsets_out[4] = BatchArr::kSup;
+  offsets_out[5] = BatchArr::kTotal;
   return EMU_SV_OK;
 }
 

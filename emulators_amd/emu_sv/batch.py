@@ -23,7 +23,7 @@ import torch
 
 from emulators_amd import _native as native
 
-PHASES = {0: "start", 1: "iterating", 2: "combining", 3: "done", 4: "failed"}
+PHASES = {0: "start", 1: "iterating", 2: "combining", 3: "done", 4: "failed", 5: "paused"}
 
 
 @dataclass
@@ -60,7 +60,7 @@ class BatchResult:
 
 class _Reg(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("steps", ctypes.c_int), ("flags", ctypes.c_int),
-                ("ham", ctypes.c_void_p), ("dt", ctypes.c_void_p), ("psi", ctypes.c_void_p),
+                ("pause_each_step", ctypes.c_int), ("ham", ctypes.c_void_p), ("dt", ctypes.c_void_p), ("psi", ctypes.c_void_p),
                 ("psi_alt", ctypes.c_void_p), ("rows", ctypes.c_void_p)]
 
 
@@ -114,7 +114,7 @@ def evolve_batch(registers: list[BatchRegister], dtype: torch.dtype = torch.comp
         rows = torch.empty((max_krylov_dim + 1) * 2**n, dtype=dtype, device=device)
         keep += [ham, dts, alt, rows]
         states.append(psi)
-        regs[i] = _Reg(n, reg.steps, native.EMU_SV_FLAG_COMPLEX if cplx else 0, ham.data_ptr(),
+        regs[i] = _Reg(n, reg.steps, native.EMU_SV_FLAG_COMPLEX if cplx else 0, 0, ham.data_ptr(),
                        dts.data_ptr(), psi.data_ptr(), alt.data_ptr(), rows.data_ptr())
     nbytes = native.LIB.emu_sv_batch_workspace_bytes(count, code, regs, tile_log, min_contig)
     if nbytes == 0:
@@ -136,6 +136,7 @@ def evolve_batch(registers: list[BatchRegister], dtype: torch.dtype = torch.comp
             native.check(native.LIB.emu_sv_batch_run(handle, todo, check_every, stream,
                                                      ctypes.byref(remaining)), "batch_run")
             done_iters += todo
+        native.check(native.LIB.emu_sv_batch_publish(handle, stream), "batch_publish")
         torch.cuda.synchronize(device)
         seconds = time.perf_counter() - t0
         phases = (ctypes.c_int * count)()

@@ -196,6 +196,7 @@ typedef struct emu_sv_batch_register {
   int n;              /* qubits (1..30) */
   int steps;          /* time steps */
   int flags;          /* EMU_SV_FLAG_COMPLEX if any step has phi != 0 */
+  int pause_each_step; /* 1: stop after every step until emu_sv_batch_resume */
   const double *ham;  /* steps * EMU_SV_HAM_DOUBLES packed blocks, one per step */
   const double *dt;   /* steps time steps in us (op = -i dt H) */
   void *psi;          /* 2^n elements: initial state in, final state out */
@@ -223,16 +224,39 @@ int emu_sv_batch_create(int count, int dtype, const emu_sv_batch_register *regs,
 
 /* Replay the batched iteration up to max_iterations times on stream, checking
  * every check_every iterations how many registers are still running
- * (*remaining_out); when none is, the final states are copied to each
- * register's psi.  Registers whose Krylov iteration did not converge within
- * max_dim stop (phase 4 in emu_sv_batch_status). */
+ * (*remaining_out; paused, done and failed registers are not running).
+ * Registers whose Krylov iteration did not converge within max_dim stop
+ * (phase 4 in emu_sv_batch_status). */
 int emu_sv_batch_run(void *handle, int max_iterations, int check_every, hipStream_t stream,
                      int *remaining_out);
+
+/* Copy the final state of every done (or failed) register into its psi
+ * (the state may sit in psi_alt after an odd number of steps). */
+int emu_sv_batch_publish(void *handle, hipStream_t stream);
 
 /* Per register (host arrays of count ints, any may be NULL): phase (0 start,
  * 1 iterating, 2 combining, 3 done, 4 failed), completed steps, matvecs. */
 int emu_sv_batch_status(void *handle, int *phase_out, int *steps_out, int *matvecs_out,
                         hipStream_t stream);
+
+/* Restart every register paused after a step (pause_each_step). */
+int emu_sv_batch_resume(void *handle, hipStream_t stream);
+
+/* The Lanczos decomposition of register reg's last finished step (valid
+ * while it is paused, or done): info_out[0..4] = phase, completed steps, m
+ * (combination coefficients / basis vectors used), 1 if the step ended in a
+ * happy breakdown, matvecs; scalars_out[0] = the norm of the step's initial
+ * state; arrays_out (host, emu_sv_batch_array_layout()[5] doubles) = T[k,k],
+ * T[k+1,k], the basis scales, the coefficients and T[k,k+1]; state_out = the
+ * current state, prev_state_out = the step's initial state (basis vector 0),
+ * rows_out = basis vectors 1.. (device pointers). */
+int emu_sv_batch_step_info(void *handle, int reg, int *info_out, double *scalars_out,
+                           double *arrays_out, void **state_out, void **prev_state_out,
+                           void **rows_out, hipStream_t stream);
+
+/* Offsets (doubles) of the arrays in emu_sv_batch_step_info's arrays_out:
+ * [T diag, T sub, scales, coefficients, T super, total]. */
+int emu_sv_batch_array_layout(int *offsets_out);
 
 int emu_sv_batch_destroy(void *handle);
 
@@ -262,6 +286,15 @@ int emu_sv_expm(int m, const double *a, double *e);
 int emu_sv_lanczos_exp(int m, const double *tdiag, const double *tsub,
                        double sigma_re, double sigma_im, double n2,
                        int want_coeffs, double *out);
+
+/* Host: eigen-decomposition J = Z diag(lambda) Z^T of the real symmetric
+ * tridiagonal J (diagonal d, m values; off-diagonal e, m-1 values) by the
+ * implicit QR iteration of emu_sv_lanczos_exp; z_out is m x m row-major
+ * (row k = component k of every eigenvector).  Used for the Frechet weights
+ * of the adjoint (the integrals of products of two Krylov trajectories)
+ * in place of a block matrix exponential (double_krylov.py:18-58). */
+int emu_sv_tridiag_eig(int m, const double *d, const double *e, double *lambda_out,
+                       double *z_out);
 
 #ifdef __cplusplus
 }

@@ -53,3 +53,41 @@ def test_frechet_weights_scalar_closed_form():
     x = frechet_weights(np.array([[a]]), np.array([[b]]))
     s = np.conj(a) + b
     assert abs(x[0, 0] - (np.exp(s) - 1.0) / s) < 1e-14
+
+
+def _lanczos_form(m: int, sigma: complex, rng) -> np.ndarray:
+    """W^-1 (sigma J) W with W = diag(w^k), w = sigma/|sigma|: the structure of
+    a Hermitian Lanczos matrix of op = sigma H (real positive subdiagonal)."""
+    j = np.real(_random_lanczos_like(m, 1.0, rng))
+    w = sigma / abs(sigma)
+    wd = w ** np.arange(m)
+    return (np.diag(1.0 / wd) @ (sigma * j) @ np.diag(wd))
+
+
+@pytest.mark.parametrize("ket_forward", [False, True])
+@pytest.mark.parametrize("mb,mk", [(1, 1), (4, 6), (9, 3), (25, 30)])
+def test_frechet_weights_eig_matches_block_exponential(ket_forward, mb, mk):
+    """The eigen-decomposition weights equal the Van Loan block-exponential
+    weights for Lanczos-structured matrices (including complex sigma)."""
+    from emulators_amd.emu_sv.adjoint import frechet_weights, frechet_weights_eig
+
+    rng = np.random.default_rng(7 * mb + mk)
+    sb = 0.3j if ket_forward else 0.25j + 0.05
+    sk = -0.3j if ket_forward else 0.25j + 0.05
+    tb = _lanczos_form(mb, sb, rng)
+    tk = _lanczos_form(mk, sk, rng)
+    ref = frechet_weights(tb, tk, ket_forward=ket_forward)
+    got = frechet_weights_eig(tb, sb, tk, sk, ket_forward=ket_forward)
+    np.testing.assert_allclose(got, ref, atol=1e-12 * max(1.0, np.abs(ref).max()), rtol=0)
+
+
+def test_lanczos_trajectory_reproduces_expm():
+    from emulators_amd.emu_sv.adjoint import lanczos_trajectory
+
+    rng = np.random.default_rng(2)
+    sigma = -0.4j
+    t = _lanczos_form(12, sigma, rng)
+    a, r = lanczos_trajectory(t, sigma)
+    for u in (0.0, 0.3, 1.0):
+        ref = scipy.linalg.expm(u * t)[:, 0]
+        np.testing.assert_allclose(a @ np.exp(u * r), ref, atol=1e-13, rtol=0)

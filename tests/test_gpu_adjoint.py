@@ -92,11 +92,14 @@ def _dense_pulse_loss(om, de, ph, u, dts, psi0):
     return torch.vdot(psi, h @ psi).real + ((psi.abs() ** 2) * pc).sum()
 
 
-@pytest.mark.parametrize("mode", ["reverse", "recompute"])
-def test_pulse_gradient_vs_dense_autograd(gpu_device, mode):
+@pytest.mark.parametrize("mode,budget", [("recompute", 1 << 34), ("recompute", 0),
+                                         ("recompute", 3 * 2**6 * 16 * 12), ("reverse", 0),
+                                         ("batched", 0)])
+def test_pulse_gradient_vs_dense_autograd(gpu_device, mode, budget):
     """Per-atom Omega/delta/phi for every step, U and psi_0 at N=6, 9 steps of
     uneven dt, checkpoints every 4 steps: against autograd through dense
-    matrix exponentials."""
+    matrix exponentials.  Recompute mode with every forward basis kept, with
+    none kept (all segments recomputed) and with only the newest kept."""
     from emulators_amd.emu_sv.adjoint import EnergyExpectation, evolve_pulse, total_occupation
 
     n, s = 6, 9
@@ -111,7 +114,7 @@ def test_pulse_gradient_vs_dense_autograd(gpu_device, mode):
 
     params = [t.clone().requires_grad_(True) for t in (om, de, ph, u)]
     p0 = psi0.to(gpu_device).requires_grad_(True)
-    psi = evolve_pulse(*params, p0, dts, 1e-12, 4, 100, mode)
+    psi = evolve_pulse(*params, p0, dts, 1e-12, 4, 100, mode, budget)
     loss = EnergyExpectation.apply(params[0][-1], params[1][-1], params[2][-1], params[3], psi) \
         + total_occupation(psi)
     loss.backward()
@@ -139,7 +142,8 @@ def _random16_tensors(steps=None):
     return torch.as_tensor(om), torch.as_tensor(de), u
 
 
-def _gpu_loss_and_grads(om_k, de_k, u, tol, mode="reverse", every=10, grads=True):
+def _gpu_loss_and_grads(om_k, de_k, u, tol, mode="recompute", every=10, grads=True,
+                        budget=1 << 34):
     from emulators_amd.emu_sv.adjoint import EnergyExpectation, evolve_pulse, total_occupation
 
     n, s = u.shape[0], om_k.shape[0]
@@ -151,7 +155,7 @@ def _gpu_loss_and_grads(om_k, de_k, u, tol, mode="reverse", every=10, grads=True
     psi0 = torch.zeros(2**n, dtype=torch.complex128, device="cuda")
     psi0[0] = 1.0
     dts = torch.full((s,), 0.01, dtype=torch.float64)
-    psi = evolve_pulse(omegas, deltas, phis, u, psi0, dts, tol, every, 100, mode)
+    psi = evolve_pulse(omegas, deltas, phis, u, psi0, dts, tol, every, 100, mode, budget)
     loss = EnergyExpectation.apply(omegas[-1], deltas[-1], phis[-1], u, psi) + total_occupation(psi)
     if grads:
         loss.backward()
@@ -196,15 +200,16 @@ def test_random16_gradient_vs_reference_autograd(gpu_device):
 
 def test_random16_gradient_vs_central_differences(gpu_device):
     """The full 16-atom, 100-step, 200-parameter workload: adjoint (checkpoints
-    every 10 steps, both modes) against central differences with h = 1e-6
-    (complex128) on Omega and delta at the first, middle and last step."""
+    every 10 steps, all three backward modes) against central differences with
+    h = 1e-6 (complex128) on Omega and delta at the first, middle and last step."""
     om_k, de_k, u = _random16_tensors()
     tol = 1e-12
-    _, g_om, g_de = _gpu_loss_and_grads(om_k, de_k, u, tol, "reverse")
-    _, g_om2, g_de2 = _gpu_loss_and_grads(om_k, de_k, u, tol, "recompute")
+    _, g_om, g_de = _gpu_loss_and_grads(om_k, de_k, u, tol, "recompute")
     scale = max(float(g_om.abs().max()), float(g_de.abs().max()))
-    assert float((g_om - g_om2).abs().max()) < 1e-7 * scale
-    assert float((g_de - g_de2).abs().max()) < 1e-7 * scale
+    for mode in ("reverse", "batched", "recompute"):
+        _, g_om2, g_de2 = _gpu_loss_and_grads(om_k, de_k, u, tol, mode, budget=0)
+        assert float((g_om - g_om2).abs().max()) < 1e-7 * scale, mode
+        assert float((g_de - g_de2).abs().max()) < 1e-7 * scale, mode
     h = 1e-6
     for which, k in (("om", 0), ("om", 50), ("om", 99), ("de", 0), ("de", 50), ("de", 99)):
         vals = []
