@@ -56,6 +56,9 @@ SIGNATURES = {
     "emu_sv_lanczos_update": (_i, [_i, _i, _vp, _vp, _d, _d, _d, _vp, _d, _d, _d, _vp, _vp, _vp]),
     "emu_sv_lanczos_matvec": (_i, [_i, _i, _vp, _d, _d, _vp, _d, _vp, _d, _d, _d, _vp, _d, _d, _d,
                                    _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
+    "emu_sv_lanczos_matvec_dev": (_i, [_i, _i, _vp, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                       _i, _i, _i, _vp]),
+    "emu_sv_lanczos_next_coef": (_i, [_vp, _vp, _d, _d, _vp, _vp, _vp, _vp]),
     "emu_sv_lincomb": (_i, [_i, _i, _i, ctypes.POINTER(_vp), _dp, _vp, _vp]),
     "emu_sv_dot": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp]),
     "emu_sv_scale": (_i, [_i, _i, _vp, _d, _vp]),
@@ -238,6 +241,29 @@ def lanczos_matvec(n: int, ham_dev: torch.Tensor, sigma: complex, w: torch.Tenso
                                     _ptr(out[0:5]), _ptr(out[5:12]), _ptr(ws.scratch), tile_log,
                                     min_contig, flags, _stream(w.device)), "lanczos_matvec")
     return out
+
+
+def lanczos_matvec_dev(n: int, ham_dev: torch.Tensor, sigma: complex, w: torch.Tensor,
+                       pa: torch.Tensor | None, pb: torch.Tensor, coef: torch.Tensor, r: torch.Tensor,
+                       rec: torch.Tensor, complex_coef: bool = True, tile_log: int = 0,
+                       min_contig: int = 0, partners: int = 0, first_log: int = 0) -> None:
+    """lanczos_matvec with device coefficients coef (6 doubles: a*pa_scale,
+    b*pb_scale, pb_scale, w_scale) for the tile kernel; rec (device, >= 12
+    doubles) receives [||z||^2, <z|pb_s>, <z|w_s>, ||r||^2, <z|r>, ...]."""
+    flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags("tile", partners, first_log)
+    check(LIB.emu_sv_lanczos_matvec_dev(n, dtype_code(w.dtype), _ptr(ham_dev), float(sigma.real),
+                                        float(sigma.imag), _ptr(w), _ptr(pa), _ptr(pb), _ptr(coef),
+                                        _ptr(r), _ptr(rec[0:5]), _ptr(rec[5:12]), _ptr(
+                                            Workspace.get(w.device, n).scratch), tile_log,
+                                        min_contig, flags, _stream(w.device)), "lanczos_matvec_dev")
+
+
+def lanczos_next_coef(rec: torch.Tensor, sigma: complex, prev: torch.Tensor, nxt: torch.Tensor,
+                      tout: torch.Tensor) -> None:
+    """Device coefficients of the next iteration from rec (see lanczos_matvec_dev)."""
+    check(LIB.emu_sv_lanczos_next_coef(_ptr(rec[0:5]), _ptr(rec[5:8]), float(sigma.real),
+                                       float(sigma.imag), _ptr(prev), _ptr(nxt), _ptr(tout),
+                                       _stream(rec.device)), "lanczos_next_coef")
 
 
 def lanczos_update(n: int, w: torch.Tensor, pa: torch.Tensor | None, pa_scale: float, a: complex,

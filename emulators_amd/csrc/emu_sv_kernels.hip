@@ -1157,6 +1157,7 @@ struct OrthoIn {
   const void *pa, *pb;
   double ca_re, ca_im, cb_re, cb_im, pb_scale;
   double *norm2_out;  // kOrthoSums doubles
+  const double *dcoef = nullptr;  // device coefficients (tile kernel only)
 };
 
 template <typename T>
@@ -1364,6 +1365,7 @@ int run_tile(int n, const double *ham, double sr, double si, const void *v, doub
     a.cb_re = ortho ? ortho->cb_re : 0.0;
     a.cb_im = ortho ? ortho->cb_im : 0.0;
     a.pb_scale = ortho ? ortho->pb_scale : 0.0;
+    a.dcoef = (ortho != nullptr && p == 0) ? ortho->dcoef : nullptr;
     a.v_scale = (ortho != nullptr && p > 0) ? 1.0 : vsc;  // later passes read z = v_scale*w already
     a.prev_scale = psc;
     a.sig_re = sr;
@@ -1753,6 +1755,45 @@ int emu_sv_lanczos_matvec(int n, int dtype, const double *ham, double sigma_re, 
                             workspace, kind, tile_log, min_contig, cplx, stream, &o);
   return run_hpsi<float>(n, ham, sigma_re, sigma_im, w, w_scale, nullptr, 0.0, r, dots_out,
                          workspace, kind, tile_log, min_contig, cplx, stream, &o);
+}
+
+int emu_sv_lanczos_matvec_dev(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
+                              void *w, const void *pa, const void *pb, const double *coef, void *r,
+                              double *norm2_out, double *dots_out, void *workspace, int tile_log,
+                              int min_contig, int flags, hipStream_t stream) {
+  if (int rc = check_n(n)) return rc;
+  if (int rc = check_dtype(dtype)) return rc;
+  if (!ham || !w || !r || !coef || !norm2_out || !dots_out || !workspace)
+    return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: null pointer");
+  if (w == r || w == pa || w == pb || r == pa || r == pb)
+    return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: w, r, pa and pb must be distinct buffers");
+  if (kernel_kind(flags) != kKernelTile)
+    return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: needs the tile kernel kind");
+  int tla = tile_first_log(flags, 0);
+  const int rc = check_tile_schedule(n, dtype, tile_log, min_contig, tla, "lanczos_matvec_dev");
+  if (rc < 0) return rc;
+  if (rc > 0) return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: fewer qubits than one tile");
+  OrthoIn o;
+  o.pa = pa;
+  o.pb = pb;
+  o.ca_re = o.ca_im = o.cb_re = o.cb_im = 0.0;
+  o.pb_scale = 0.0;
+  o.norm2_out = norm2_out;
+  o.dcoef = coef;
+  const bool cplx = (flags & EMU_SV_FLAG_COMPLEX) != 0;
+  if (dtype == EMU_SV_C128)
+    return run_tile<double>(n, ham, sigma_re, sigma_im, w, 1.0, nullptr, 0.0, r, dots_out, workspace,
+                            tla, tile_log, min_contig, tile_partners(flags), cplx, stream, &o);
+  return run_tile<float>(n, ham, sigma_re, sigma_im, w, 1.0, nullptr, 0.0, r, dots_out, workspace, tla,
+                         tile_log, min_contig, tile_partners(flags), cplx, stream, &o);
+}
+
+int emu_sv_lanczos_next_coef(const double *norm2, const double *dots, double sigma_re, double sigma_im,
+                             const double *prev, double *next, double *t_out, hipStream_t stream) {
+  if (!norm2 || !dots || !prev || !next || !t_out) return fail(EMU_SV_EINVAL, "lanczos_next_coef: null");
+  hipLaunchKernelGGL(lanczos_next_coef_kernel, dim3(1), dim3(64), 0, stream, norm2, dots, sigma_re,
+                     sigma_im, prev, next, t_out);
+  return hip_check(hipGetLastError(), "lanczos_next_coef launch");
 }
 
 int emu_sv_lanczos_update(int n, int dtype, void *w, const void *pa, double pa_scale, double a_re,

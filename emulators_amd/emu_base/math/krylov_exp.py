@@ -34,6 +34,7 @@ Two implementations sit behind it:
 from __future__ import annotations
 
 import math
+import os
 from dataclasses import dataclass
 from typing import Callable
 
@@ -231,6 +232,153 @@ def _fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
     return finish(max_krylov_dim, T[max_krylov_dim, max_krylov_dim - 1].real, False, False)
 
 
+class _PipelineBuffers:
+    """Per-device records of the pipelined engine: for iteration j, device
+    row j holds [0:12] the reductions, [16:22] the coefficients it reads,
+    [24:30] the coefficients' side outputs (n2, b, a, ww of j+1); a pinned
+    host mirror receives rows as they complete."""
+
+    _cache: dict = {}
+    WIDTH = 32
+
+    def __init__(self, device: torch.device, rows: int):
+        self.dev = torch.zeros((rows, self.WIDTH), dtype=torch.float64, device=device)
+        self.host = torch.zeros((rows, self.WIDTH), dtype=torch.float64, pin_memory=True)
+        self.rows = rows
+
+    @classmethod
+    def get(cls, device: torch.device, rows: int) -> "_PipelineBuffers":
+        key = device.index if device.index is not None else torch.cuda.current_device()
+        b = cls._cache.get(key)
+        if b is None or b.rows < rows:
+            b = _PipelineBuffers(device, rows)
+            cls._cache[key] = b
+        return b
+
+
+def _pipelined_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
+                       norm_tolerance: float, max_krylov_dim: int, sched,
+                       rows: torch.Tensor | None = None) -> LanczosRun:
+    """The arithmetic of _fused_lanczos with the host one iteration behind the
+    device: the coefficients of iteration j+1 are formed on the device
+    (emu_sv_lanczos_next_coef) right after iteration j, so iteration j+1 is
+    enqueued before the host has read iteration j's reductions; the host
+    checks convergence of iteration j while j+1 runs.  The iteration in
+    flight when convergence is found is discarded (one more matvec per step
+    than _fused_lanczos), and its T entries come from the same device values
+    the kernels used."""
+    ham = op.hamiltonian
+    n = native.nqubits_of(v)
+    params = ham.device_params(v.device)
+    KrylovStats.calls += 1
+    if rows is not None:
+        if rows.dim() != 2 or rows.shape[0] < max_krylov_dim + 2 or rows.shape[1] != v.numel() \
+                or rows.dtype != v.dtype or not rows.is_contiguous():
+            raise ValueError("rows must be a contiguous (max_krylov_dim + 2, 2**n) tensor of v's dtype")
+        rows[0].copy_(v)
+        v = rows[0]
+
+    def new_vec(k: int) -> torch.Tensor:
+        return rows[k] if rows is not None else torch.empty_like(v)
+
+    buf = _PipelineBuffers.get(v.device, max_krylov_dim + 2)
+    dev, host = buf.dev, buf.host
+    stream = torch.cuda.current_stream(v.device)
+    sig = op.sigma
+    kw = dict(tile_log=sched.tile_log, min_contig=sched.min_contig, partners=sched.partners,
+              first_log=sched.first_log)
+
+    initial_norm = math.sqrt(native.dot(n, v)[0].item())
+    basis = [v]
+    scales = [1.0 / initial_norm]
+    T = np.zeros((max_krylov_dim + 2, max_krylov_dim + 2), dtype=np.complex128)
+
+    def finish(m: int, n2: float, happy: bool, converged: bool) -> LanczosRun:
+        _, _, c = native.lanczos_exp(T, m, sig, n2, True)
+        c = np.asarray(c[:m] if happy else c[: m + 1]).copy()
+        return LanczosRun(basis, scales, T[:m, :m].copy(), c, initial_norm, converged, happy, m,
+                          0.0 if happy else n2)
+
+    wbuf = new_vec(1)
+    d = native.hpsi(n, params, sig, v, scales[0], wbuf, dots=True, complex_coef=ham.complex,
+                    kernel="tile", **kw).cpu().numpy()
+    KrylovStats.matvecs += 1
+    b0 = complex(d[1], d[2])
+    T[0, 0] = b0
+    ww = d[0]  # ||op(v_0)||^2 for iteration 0's error estimate
+    # coefficients of iteration 0: a = 0, b = b0, pb = v (scale s0), w_scale = 1
+    host[0, 16:22] = torch.tensor([0.0, 0.0, b0.real * scales[0], b0.imag * scales[0], scales[0], 1.0],
+                                  dtype=torch.float64)
+    dev[0, 16:22].copy_(host[0, 16:22], non_blocking=True)
+    events = []
+    outs = []  # r buffers
+
+    def enqueue(j: int) -> None:
+        r = new_vec(j + 2)
+        outs.append(r)
+        w = wbuf_of(j)
+        native.lanczos_matvec_dev(n, params, sig, w, basis_at(j - 1) if j > 0 else None, basis_at(j),
+                                  dev[j, 16:22], r, dev[j, 0:12], complex_coef=ham.complex, **kw)
+        native.lanczos_next_coef(dev[j, 0:12], sig, dev[j, 16:22], dev[j + 1, 16:22], dev[j, 24:30])
+        host[j].copy_(dev[j], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(stream)
+        events.append(ev)
+        KrylovStats.matvecs += 1
+
+    # buffers: iteration j reads w_j (basis j+1 before orthogonalisation), writes r_j
+    def wbuf_of(j: int) -> torch.Tensor:
+        return wbuf if j == 0 else outs[j - 1]
+
+    def basis_at(k: int) -> torch.Tensor:
+        return v if k == 0 else wbuf_of(k - 1)
+
+    enqueue(0)
+    for j in range(max_krylov_dim):
+        if j + 1 < max_krylov_dim:
+            enqueue(j + 1)
+        events[j].synchronize()
+        rec = host[j].numpy()
+        nrm = math.sqrt(max(ww, 0.0))
+        n2 = float(rec[24])
+        T[j + 1, j] = n2
+        if n2 < norm_tolerance:
+            return finish(j + 1, 0.0, True, True)
+        basis.append(wbuf_of(j))
+        scales.append(1.0 / n2)
+        T[j + 2, j + 1] = 1.0
+        e1, e2, _ = native.lanczos_exp(T, j + 1, sig, n2, False)
+        err2 = e2 * nrm
+        err = e1 if e1 < err2 else (e1 * err2 / (e1 - err2))
+        if err < exp_tolerance:
+            return finish(j + 1, n2, False, True)
+        T[j + 1, j + 1] = complex(rec[25], rec[26])
+        T[j, j + 1] = complex(rec[27], rec[28])
+        ww = float(rec[29])
+    return finish(max_krylov_dim, T[max_krylov_dim, max_krylov_dim - 1].real, False, False)
+
+
+PIPELINE = os.environ.get("EMU_SV_PIPELINE", "1") != "0"
+
+
+def fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
+                  norm_tolerance: float, max_krylov_dim: int,
+                  rows: torch.Tensor | None = None) -> LanczosRun:
+    """The fused engine: pipelined (device-side coefficients) when the tuned
+    schedule is the tile kernel and N covers its tiles, else one read-back
+    per iteration.  EMU_SV_PIPELINE=0 turns the pipeline off."""
+    ham = op.hamiltonian
+    n = native.nqubits_of(v)
+    if n == ham.nqubits:
+        sched = autotune.best_schedule(n, v.dtype, v.device, ham.device_params(v.device), ham.complex)
+        tl = sched.tile_log or (13 if v.dtype == torch.complex64 else 12)
+        fl = sched.first_log or tl
+        if PIPELINE and sched.kernel == "tile" and n >= max(tl, fl):
+            return _pipelined_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim, sched,
+                                      rows)
+    return _fused_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim, rows)
+
+
 def _generic_krylov(op: Callable, v: torch.Tensor, is_hermitian: bool, exp_tolerance: float,
                     norm_tolerance: float, max_krylov_dim: int) -> KrylovExpResult:
     initial_norm = v.norm()
@@ -275,7 +423,7 @@ def krylov_exp_impl(op: Callable[[torch.Tensor], torch.Tensor], v: torch.Tensor,
     if isinstance(op, ScaledHamiltonian) and is_hermitian:
         if not v.is_cuda:
             raise ValueError("the fused Krylov engine needs the state on the GPU")
-        run = _fused_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim)
+        run = fused_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim)
         return KrylovExpResult(run.combine(), run.converged, run.happy_breakdown,
                                run.iteration_count)
     return _generic_krylov(op, v, is_hermitian, exp_tolerance, norm_tolerance, max_krylov_dim)

@@ -52,8 +52,8 @@ from emulators_amd.emu_base.math.krylov_exp import (
     DEFAULT_MAX_KRYLOV_DIM,
     LanczosRun,
     ScaledHamiltonian,
-    _fused_lanczos,
 )
+from emulators_amd.emu_base.math.krylov_exp import fused_lanczos as _fused_lanczos
 from emulators_amd.emu_sv.hamiltonian import RydbergHamiltonian
 
 

@@ -137,6 +137,29 @@ int emu_sv_lanczos_matvec(int n, int dtype, const double *ham, double sigma_re,
                           double *dots_out, void *workspace, int tile_log,
                           int min_contig, int flags, hipStream_t stream);
 
+/* emu_sv_lanczos_matvec with its coefficients in device memory, so that a
+ * caller can enqueue iteration j+1 before reading iteration j's results:
+ * coef = {Re, Im of a*pa_scale, Re, Im of b*pb_scale, pb_scale, w_scale}
+ * (6 doubles, device), as written by emu_sv_lanczos_next_coef.  Tile kernel
+ * kind only (flags), n >= the tile sizes.  Same outputs as
+ * emu_sv_lanczos_matvec (krylov_exp.py:50-61). */
+int emu_sv_lanczos_matvec_dev(int n, int dtype, const double *ham, double sigma_re,
+                              double sigma_im, void *w, const void *pa, const void *pb,
+                              const double *coef, void *r, double *norm2_out,
+                              double *dots_out, void *workspace, int tile_log,
+                              int min_contig, int flags, hipStream_t stream);
+
+/* On the device (one thread): the coefficients of the next fused Lanczos
+ * iteration from the reductions of the current one (norm2 = the 5
+ * orthogonalisation sums, dots = ||r||^2, <z|r>) and prev (the current
+ * coefficient record, whose pb_scale becomes the next pa scale):
+ *   n2 = ||z||, a = sigma conj(<z|w_s>)/(n2 conj(sigma)),
+ *   b = <z|r>/n2^2 - a <z|pb_s>/n2        (krylov_exp.py:54-61 for j+1)
+ * next = {a*s_j, b/n2, 1/n2, 1/n2}; t_out = {n2, b, a, ||r||^2/n2^2}. */
+int emu_sv_lanczos_next_coef(const double *norm2, const double *dots, double sigma_re,
+                             double sigma_im, const double *prev, double *next,
+                             double *t_out, hipStream_t stream);
+
 /* w <- w - a*(pa_scale*pa) - b*(pb_scale*pb); norm2_out[0] = ||w||^2.
  * The orthogonalisation of krylov_exp.py:54-61 (two-vector MGS, with the
  * coefficients computed by emu_sv_hpsi's fused dots). pa/pb may be NULL. */
