@@ -165,6 +165,13 @@ def cpu_baseline(seq, mean_iters, step, sample_iters):
         log(f"CPU baseline: Lanczos iteration done at {time.perf_counter() - t0:.1f} s")
     t_iter = (time.perf_counter() - t0) / sample_iters
     t_step = t_diag + mean_iters * t_iter
+    # the N=10 chain (100 steps) run whole on the oracle: the reference CPU path's own workload
+    from emulators_amd import workloads
+    seq10 = workloads.chain10()
+    t0 = time.perf_counter()
+    orc.run_sequence(seq10.omega, seq10.delta, seq10.phi, seq10.interaction_matrix, seq10.target_times)
+    t_chain10 = time.perf_counter() - t0
+    log(f"CPU baseline: N=10 chain (100 steps) on the oracle in {t_chain10:.2f} s")
     return {
         "value": 1.0 / t_step,
         "unit": "Krylov steps/s",
@@ -174,6 +181,7 @@ def cpu_baseline(seq, mean_iters, step, sample_iters):
                    f"Hamiltonian of step {step}: diagonal build {t_diag:.2f} s + "
                    f"{sample_iters} Lanczos iterations at {t_iter:.2f} s each, scaled to "
                    f"{mean_iters:.1f} iterations per step (the GPU run's mean)"),
+        "chain10_seconds": t_chain10,
     }
 
 
@@ -313,6 +321,64 @@ def batch_throughput(sample_every: int = 16) -> dict:
     return out
 
 
+def parity_legs() -> dict:
+    """The two parity workloads, timed on the GPU with their accuracy stated.
+
+    chain10: N=10 chain, 5 um, Omega = 4 pi, delta -10 -> 10 rad/us over 1000 ns
+    (100 steps), |g..g>, occupations and energy at the end; complex128 and
+    complex64 against the committed complex128 fixture tests/golden/chain10.npz
+    (written by the oracle, cross-checked by dense eigendecomposition steps).
+    square20: 4x5 lattice, 6 um, same pulse over 2000 ns (200 steps);
+    complex64 (FP64 Lanczos scalars) against the complex128 GPU run."""
+    import numpy as np
+    import torch
+
+    from emulators_amd import workloads
+    from emulators_amd.emu_sv import Energy, Occupation, StateResult, SVBackend, SVConfig
+
+    def run(seq, precision):
+        cfg = SVConfig(observables=[StateResult(evaluation_times=[1.0]),
+                                    Occupation(evaluation_times=[1.0]), Energy(evaluation_times=[1.0])],
+                       log_level=40, precision=precision)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = SVBackend._run_from_sequence_data(seq, cfg)
+        torch.cuda.synchronize()
+        return res, time.perf_counter() - t0
+
+    def compare(psi, occ, en, ref_psi, ref_occ, ref_en):
+        a = psi.to(torch.complex128)
+        b = ref_psi.to(torch.complex128)
+        fid = float(torch.abs(torch.vdot(a, b)) ** 2 / (torch.vdot(a, a).real * torch.vdot(b, b).real))
+        return {"fidelity_deficit": 1.0 - fid,
+                "max_occupation_error": float(torch.max(torch.abs(occ - ref_occ))),
+                "energy_error": abs(float(en) - float(ref_en))}
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "chain10.npz")
+    with np.load(path, allow_pickle=False) as f:
+        gold = {k: f[k] for k in f.files}
+    seq = workloads.chain10()
+    out = {"chain10": {}, "square20": {}}
+    run(seq, "complex128")  # warm
+    for prec in ("complex128", "complex64"):
+        res, t = run(seq, prec)
+        out["chain10"][prec] = {"seconds": t, **compare(
+            res.state[-1].data.cpu(), res.occupation[-1].cpu(), res.energy[-1],
+            torch.as_tensor(gold["final_state"]), torch.as_tensor(gold["occupation"]), gold["energy"])}
+    seq = workloads.square20()
+    r128, t128 = run(seq, "complex128")
+    r64, t64 = run(seq, "complex64")
+    out["square20"] = {"seconds_c128": t128, "seconds_c64": t64,
+                       "c64_vs_c128": compare(r64.state[-1].data.cpu(), r64.occupation[-1].cpu(),
+                                              r64.energy[-1], r128.state[-1].data.cpu(),
+                                              r128.occupation[-1].cpu(), r128.energy[-1])}
+    c = out["chain10"]["complex128"]
+    log(f"parity: chain10 c128 {c['seconds']:.2f} s, 1-F {c['fidelity_deficit']:.1e}, "
+        f"occupation err {c['max_occupation_error']:.1e}; square20 c64 vs c128 1-F "
+        f"{out['square20']['c64_vs_c128']['fidelity_deficit']:.1e}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -325,6 +391,7 @@ def main() -> int:
     ap.add_argument("--duration", type=int, default=3000, help="pulse duration in ns")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--only-batch", action="store_true", help="time only the batching workload")
     ap.add_argument("--only-adjoint", action="store_true", help="time only the gradient workload")
     ap.add_argument("--only-matvec", action="store_true",
@@ -403,6 +470,7 @@ def main() -> int:
 
     adjoint = None if args.no_adjoint else adjoint_cost()
     batch = None if args.no_batch else batch_throughput()
+    parity = None if args.no_parity else parity_legs()
     mean_iters = mv128 / steps
     cpu = None
     if not args.no_cpu_baseline:
@@ -448,6 +516,7 @@ def main() -> int:
         "hpsi": hpsi,
         "adjoint": adjoint,
         "batch": batch,
+        "parity": parity,
     }
     print(json.dumps(out), flush=True)
     return 0
