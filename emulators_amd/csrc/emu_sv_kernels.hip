@@ -1420,6 +1420,9 @@ struct BatchLayout {
 int batch_plan(const emu_sv_batch_register &r, int dtype, int tl, int cmin, BatchReg &out) {
   if (r.n < 1 || r.n > 30 || r.steps < 1 || !r.ham || !r.dt || !r.psi || !r.psi_alt || !r.rows)
     return fail(EMU_SV_EINVAL, "batch: bad register description");
+  if ((reinterpret_cast<uintptr_t>(r.psi) | reinterpret_cast<uintptr_t>(r.psi_alt) |
+       reinterpret_cast<uintptr_t>(r.rows)) & 15)
+    return fail(EMU_SV_EINVAL, "batch: psi, psi_alt and rows must be 16-byte aligned");
   const int t = r.n < tl ? r.n : tl;
   Pass plan[EMU_SV_MAX_QUBITS];
   const int np = make_plan(r.n, t, cmin, plan);
@@ -1979,9 +1982,11 @@ int emu_sv_batch_create(int count, int dtype, const emu_sv_batch_register *regs,
   if (!handle_out) return fail(EMU_SV_EINVAL, "batch_create: null handle pointer");
   *handle_out = nullptr;
   if (count < 1 || !regs || !workspace) return fail(EMU_SV_EINVAL, "batch_create: bad arguments");
+  if (reinterpret_cast<uintptr_t>(workspace) & 255)
+    return fail(EMU_SV_EINVAL, "batch_create: workspace must be 256-byte aligned");
   if (int rc = check_dtype(dtype)) return rc;
   if (max_dim < 2 || max_dim > kBatchMaxDim)
-    return fail(EMU_SV_EINVAL, "batch_create: max_dim must be in [2, 63]");
+    return fail(EMU_SV_EINVAL, "batch_create: max_dim must be in [2, 62]");
   if (!(tolerance > 0.0)) return fail(EMU_SV_EINVAL, "batch_create: tolerance must be > 0");
   if (int rc = batch_tile_logs(dtype, tile_log, min_contig)) return rc;
   auto *h = new BatchHandle();

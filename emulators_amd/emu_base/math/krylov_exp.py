@@ -263,10 +263,10 @@ def _pipelined_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: fl
     device: the coefficients of iteration j+1 are formed on the device
     (emu_sv_lanczos_next_coef) right after iteration j, so iteration j+1 is
     enqueued before the host has read iteration j's reductions; the host
-    checks convergence of iteration j while j+1 runs.  The iteration in
-    flight when convergence is found is discarded (one more matvec per step
-    than _fused_lanczos), and its T entries come from the same device values
-    the kernels used."""
+    checks convergence of iteration j while j+1 runs.  When the error
+    sequence predicts that iteration j converges, j+1 is not enqueued ahead;
+    otherwise the iteration in flight when convergence is found is discarded.
+    T's entries come from the same device values the kernels used."""
     ham = op.hamiltonian
     n = native.nqubits_of(v)
     params = ham.device_params(v.device)
@@ -334,8 +334,14 @@ def _pipelined_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: fl
         return v if k == 0 else wbuf_of(k - 1)
 
     enqueue(0)
+    errs: list = []
     for j in range(max_krylov_dim):
-        if j + 1 < max_krylov_dim:
+        # Speculate on iteration j+1 unless the error sequence predicts that
+        # iteration j converges (geometric extrapolation with a 10x margin);
+        # a wrong guess only costs the overlap of one iteration.
+        likely_done = len(errs) >= 2 and errs[-2] > 0.0 and \
+            errs[-1] * (errs[-1] / errs[-2]) < 0.1 * exp_tolerance
+        if j + 1 < max_krylov_dim and not likely_done:
             enqueue(j + 1)
         events[j].synchronize()
         rec = host[j].numpy()
@@ -352,9 +358,12 @@ def _pipelined_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: fl
         err = e1 if e1 < err2 else (e1 * err2 / (e1 - err2))
         if err < exp_tolerance:
             return finish(j + 1, n2, False, True)
+        errs.append(err)
         T[j + 1, j + 1] = complex(rec[25], rec[26])
         T[j, j + 1] = complex(rec[27], rec[28])
         ww = float(rec[29])
+        if j + 1 < max_krylov_dim and len(events) == j + 1:
+            enqueue(j + 1)  # the speculation was skipped and j did not converge
     return finish(max_krylov_dim, T[max_krylov_dim, max_krylov_dim - 1].real, False, False)
 
 

@@ -276,7 +276,7 @@ def adjoint_sweep(pulse: _Pulse, checkpoints: dict, psi_final: torch.Tensor, lam
                   stored: dict | None = None):
     """Reverse sweep; returns per-step parameter gradients and dL/dpsi_0."""
     if mode == "batched":
-        return _adjoint_sweep_batched(pulse, checkpoints, psi_final, lam, tol, min(max_dim, 63),
+        return _adjoint_sweep_batched(pulse, checkpoints, psi_final, lam, tol, min(max_dim, 62),
                                       want_u)
     s, n = pulse.steps, pulse.n
     g_om = np.zeros((s, n))

@@ -214,7 +214,8 @@ int emu_sv_parameter_traces(int n, int dtype, int count, const void *w,
 
 /* ---- many registers in one device-driven engine ------------------------ */
 /* One register of a batch.  All pointers are device memory owned by the
- * caller.  psi holds the initial state and receives the final state. */
+ * caller and 16-byte aligned.  psi holds the initial state and receives the
+ * final state. */
 typedef struct emu_sv_batch_register {
   int n;              /* qubits (1..30) */
   int steps;          /* time steps */
@@ -236,9 +237,9 @@ size_t emu_sv_batch_workspace_bytes(int count, int dtype, const emu_sv_batch_reg
  * the Expokit test of emu_sv_lanczos_exp) step after step
  * (emu_sv/sv_backend_impl.py:186-209), with its whole state machine in device
  * memory, so one batched iteration is a fixed kernel sequence (captured once
- * in a hipGraph).  max_dim <= 63; tile_log 8..12 / min_contig: the H.psi pass
+ * in a hipGraph).  max_dim <= 62; tile_log 8..12 / min_contig: the H.psi pass
  * split (0 = default).  The library allocates no device memory: the
- * workspace (zero-filled not required) is the caller's.  *handle_out is a
+ * workspace (256-byte aligned, zero-filled not required) is the caller's.  *handle_out is a
  * host object released by emu_sv_batch_destroy.  No counterpart in the
  * reference (it evolves one register per call). */
 int emu_sv_batch_create(int count, int dtype, const emu_sv_batch_register *regs, int max_dim,
