@@ -23,6 +23,7 @@ EMU_SV_ENOTCONVERGED = -3
 EMU_SV_C128 = 0
 EMU_SV_C64 = 1
 EMU_SV_FLAG_COMPLEX = 1
+EMU_SV_FLAG_NOISE = 2
 EMU_SV_KERNEL_DEFAULT = 0
 EMU_SV_KERNEL_LDS = 1 << 8
 EMU_SV_KERNEL_WAVE = 2 << 8
@@ -39,7 +40,7 @@ def kernel_flags(kernel: str, partners: int = 0, first_log: int = 0) -> int:
         raise ValueError("partners and first_log must be in [0, 15]")
     return KERNELS[kernel] | (partners << 12) | (first_log << 16)
 EMU_SV_MAX_QUBITS = 32
-EMU_SV_HAM_DOUBLES = 1128
+EMU_SV_HAM_DOUBLES = 1136
 
 # Every symbol declared in include/emu_sv_hip.h, with (restype, argtypes).
 _vp = ctypes.c_void_p
@@ -62,6 +63,10 @@ SIGNATURES = {
     "emu_sv_lincomb": (_i, [_i, _i, _i, ctypes.POINTER(_vp), _dp, _vp, _vp]),
     "emu_sv_dot": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp]),
     "emu_sv_scale": (_i, [_i, _i, _vp, _d, _vp]),
+    "emu_sv_pack_noise": (_i, [_dp, _dp]),
+    "emu_sv_arnoldi_orthogonalize": (_i, [_i, _i, _i, ctypes.POINTER(_vp), _dp, _vp, _vp, _vp,
+                                          _vp]),
+    "emu_sv_apply_1q": (_i, [_i, _i, _vp, _i, _dp, _vp]),
     "emu_sv_convert": (_i, [_i, _i, _vp, _i, _vp, _vp]),
     "emu_sv_correlation": (_i, [_i, _i, _vp, _vp, _vp, _vp]),
     "emu_sv_parameter_traces": (_i, [_i, _i, _i, _vp, _vp, ctypes.c_uint64, _i, _vp, _vp, _vp]),
@@ -166,6 +171,22 @@ def pack_hamiltonian(omega, delta, phi, interaction) -> tuple[np.ndarray, bool]:
     return block, bool(np.any(ph != 0.0))
 
 
+def _complex_2x2(a) -> np.ndarray:
+    m = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+    if m.shape != (2, 2):
+        raise ValueError("expected a 2x2 complex matrix")
+    return np.ascontiguousarray(m.reshape(4)).view(np.float64)
+
+
+def pack_noise(block: np.ndarray, noise) -> np.ndarray:
+    """Add the single-qubit term K = -i/2 sum L^+L (2x2 complex) to a packed
+    block (host, in place)."""
+    k = _complex_2x2(noise)
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(LIB.emu_sv_pack_noise(k.ctypes.data_as(dp), block.ctypes.data_as(dp)), "pack_noise")
+    return block
+
+
 def plan_passes(n: int, dtype: torch.dtype, tile_log: int = 0, min_contig: int = 0,
                 kernel: str = "default", partners: int = 0, first_log: int = 0) -> int:
     rc = LIB.emu_sv_plan_passes(n, dtype_code(dtype), tile_log, min_contig,
@@ -201,8 +222,9 @@ def hpsi(n: int, ham_dev: torch.Tensor, sigma: complex, v: torch.Tensor, v_scale
          w: torch.Tensor, prev: torch.Tensor | None = None, prev_scale: float = 1.0,
          dots: bool = False, complex_coef: bool = True, tile_log: int = 0,
          min_contig: int = 0, kernel: str = "default", partners: int = 0,
-         first_log: int = 0) -> torch.Tensor | None:
-    """w = sigma * H (v_scale v); optionally returns the 7 fused dots (device)."""
+         first_log: int = 0, noise: bool = False) -> torch.Tensor | None:
+    """w = sigma * H (v_scale v); optionally returns the 7 fused dots (device).
+    ``noise``: the block carries the non-Hermitian term (LDS kernel)."""
     _require_device_vec(v, "hpsi")
     _require_device_vec(w, "hpsi")
     if v.dtype != w.dtype or v.numel() != w.numel() or v.numel() != (1 << n):
@@ -212,6 +234,8 @@ def hpsi(n: int, ham_dev: torch.Tensor, sigma: complex, v: torch.Tensor, v_scale
     ws = Workspace.get(v.device, n)
     out = ws.results[0:7] if dots else None
     flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags(kernel, partners, first_log)
+    if noise:
+        flags |= EMU_SV_FLAG_NOISE
     check(LIB.emu_sv_hpsi(n, dtype_code(v.dtype), _ptr(ham_dev), float(sigma.real),
                           float(sigma.imag), _ptr(v), float(v_scale), _ptr(prev),
                           float(prev_scale), _ptr(w), _ptr(out), _ptr(ws.scratch), tile_log,
@@ -304,6 +328,43 @@ def dot(n: int, a: torch.Tensor, b: torch.Tensor | None = None) -> torch.Tensor:
     check(LIB.emu_sv_dot(n, dtype_code(a.dtype), _ptr(a), _ptr(b), _ptr(out), _ptr(ws.scratch),
                          _stream(a.device)), "dot")
     return out
+
+
+def arnoldi_orthogonalize(n: int, basis: list[torch.Tensor], scales: list[float],
+                          w: torch.Tensor) -> torch.Tensor:
+    """Modified Gram-Schmidt of w against scales[k] * basis[k] on the device.
+    Returns a device (count+1, 4) float64 tensor: row k < count =
+    [||w||^2 before subtraction k, Re, Im <basis_k|w>] (unscaled), row count =
+    [||w||^2 after the last subtraction]."""
+    _require_device_vec(w, "arnoldi_orthogonalize")
+    count = len(basis)
+    if count != len(scales) or count < 1:
+        raise ValueError("arnoldi_orthogonalize: one scale per basis vector")
+    for v in basis:
+        _require_device_vec(v, "arnoldi_orthogonalize")
+        if v.dtype != w.dtype or v.numel() != w.numel() or v.numel() != (1 << n):
+            raise ValueError("arnoldi_orthogonalize: basis vectors must match w")
+    ws = Workspace.get(w.device, n)
+    rec = torch.empty(count + 1, 4, dtype=torch.float64, device=w.device)
+    arr = (ctypes.c_void_p * count)(*[v.data_ptr() for v in basis])
+    sc = np.ascontiguousarray(np.asarray(scales, dtype=np.float64))
+    check(LIB.emu_sv_arnoldi_orthogonalize(n, dtype_code(w.dtype), count, arr,
+                                           sc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                           _ptr(w), _ptr(rec), _ptr(ws.scratch),
+                                           _stream(w.device)), "arnoldi_orthogonalize")
+    return rec
+
+
+def apply_1q(n: int, psi: torch.Tensor, qubit: int, op) -> torch.Tensor:
+    """psi <- op acting on qubit ``qubit`` (bit n-1-qubit), in place."""
+    _require_device_vec(psi, "apply_1q")
+    if not 0 <= qubit < n:
+        raise ValueError("apply_1q: qubit out of range")
+    a = _complex_2x2(op.detach().cpu().numpy() if isinstance(op, torch.Tensor) else op)
+    check(LIB.emu_sv_apply_1q(n, dtype_code(psi.dtype), _ptr(psi), n - 1 - qubit,
+                              a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                              _stream(psi.device)), "apply_1q")
+    return psi
 
 
 def scale(n: int, v: torch.Tensor, s: float) -> None:

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
+#include <cmath>
 #include <complex>
 #include <mutex>
 #include <string>
@@ -68,6 +69,7 @@ constexpr int kDefaultContigC128 = 2;
 
 // Offsets into the packed parameter block (see emu_sv_hip.h).
 constexpr int kCre = 0, kCim = 32, kDelta = 64, kU = 96, kFlagComplex = 1120;
+constexpr int kFlagNoise = 1121, kNoise = 1122;  // K00, K01, K10, K11 (re, im)
 
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
@@ -174,9 +176,13 @@ __device__ __forceinline__ typename Vec2<T>::type load_input(const PassArgs &a, 
 // ---------------------------------------------------------------------------
 // One tile of a pass: tile index g, diagonal constants row[0] = E_out,
 // row[1 + p] = a_p (when a.diag), partial sums into row prow of the partials.
-template <typename T, bool CPLX, bool DOTS>
+// NOISE (with CPLX): the block's single-qubit term K is added, K10 / K01 to
+// the flip coefficients and (n - popcount(x)) K00 + popcount(x) K11 to the
+// diagonal of amplitude x.
+template <typename T, bool CPLX, bool DOTS, bool NOISE = false>
 __device__ __forceinline__ void pass_tile(const PassArgs &a, uint64_t g, const double *row,
                                           uint64_t prow) {
+  static_assert(!NOISE || CPLX, "the noise term needs complex coefficients");
   using T2 = typename Vec2<T>::type;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T2 *tile = reinterpret_cast<T2 *>(smem_raw);
@@ -248,21 +254,40 @@ __device__ __forceinline__ void pass_tile(const PassArgs &a, uint64_t g, const d
     __syncthreads();
   }
 
-  // Flip coefficients of the local bits (uniform values).
+  // Flip coefficients of the local bits (uniform values).  With NOISE the
+  // two directions differ: bit set <1|H|0> = c + K10 (cre, cim), bit clear
+  // <0|H|1> = conj(c) + K01 (dre, dim).
   uint32_t fm_local = 0;
   T cre[kMaxTileLog], cim[kMaxTileLog];
+  T dre[NOISE ? kMaxTileLog : 1], dim[NOISE ? kMaxTileLog : 1];
 #pragma unroll
   for (int i = 0; i < kMaxTileLog; ++i) {
     cre[i] = 0;
     cim[i] = 0;
+    if (NOISE) dre[i] = dim[i] = 0;
     if (i < t) {
       const int b = gbit(i);
       if ((a.flip_mask >> b) & 1u) {
         fm_local |= 1u << i;
-        cre[i] = static_cast<T>(H[kCre + b]);
-        cim[i] = static_cast<T>(H[kCim + b]);
+        if (NOISE) {
+          cre[i] = static_cast<T>(H[kCre + b] + H[kNoise + 4]);
+          cim[i] = static_cast<T>(H[kCim + b] + H[kNoise + 5]);
+          dre[i] = static_cast<T>(H[kCre + b] + H[kNoise + 2]);
+          dim[i] = static_cast<T>(-H[kCim + b] + H[kNoise + 3]);
+        } else {
+          cre[i] = static_cast<T>(H[kCre + b]);
+          cim[i] = static_cast<T>(H[kCim + b]);
+        }
       }
     }
+  }
+  // diagonal noise: n K00 + popcount(x) (K11 - K00)
+  double k0r = 0.0, k0i = 0.0, k1r = 0.0, k1i = 0.0;
+  if (NOISE) {
+    k0r = n * H[kNoise + 0];
+    k0i = n * H[kNoise + 1];
+    k1r = H[kNoise + 6] - H[kNoise + 0];
+    k1i = H[kNoise + 7] - H[kNoise + 1];
   }
   const T sr = static_cast<T>(a.sig_re), si = static_cast<T>(a.sig_im);
   T2 *__restrict__ out = reinterpret_cast<T2 *>(a.out);
@@ -279,7 +304,12 @@ __device__ __forceinline__ void pass_tile(const PassArgs &a, uint64_t g, const d
       for (int i = 0; i < kMaxTileLog; ++i) {
         if ((fm_local >> i) & 1u) {
           const T2 nb = tile[l ^ (1u << i)];
-          if (CPLX) {
+          if (NOISE) {
+            const bool up = (l >> i) & 1u;
+            const T fr = up ? cre[i] : dre[i], fi = up ? cim[i] : dim[i];
+            hr += fr * nb.x - fi * nb.y;
+            hi += fr * nb.y + fi * nb.x;
+          } else if (CPLX) {
             // bit set: <1|H|0> = c ; bit clear: <0|H|1> = conj(c)
             const T ci = ((l >> i) & 1u) ? cim[i] : -cim[i];
             hr += cre[i] * nb.x - ci * nb.y;
@@ -290,18 +320,26 @@ __device__ __forceinline__ void pass_tile(const PassArgs &a, uint64_t g, const d
           }
         }
       }
+      const uint64_t x = gidx(l);
       if (a.diag) {
         double e = dbase + s_B[r];
 #pragma unroll
         for (int j = 0; j < kMaxTileLog - kThreadBits; ++j)
           if ((r >> j) & 1u) e += Gj[j];
-        hr += static_cast<T>(e * static_cast<double>(psi.x));
-        hi += static_cast<T>(e * static_cast<double>(psi.y));
+        if (NOISE) {
+          const double pc = (double)__popcll(x);
+          const double er = e + k0r + pc * k1r, ei = k0i + pc * k1i;
+          const double pr = static_cast<double>(psi.x), pi = static_cast<double>(psi.y);
+          hr += static_cast<T>(er * pr - ei * pi);
+          hi += static_cast<T>(er * pi + ei * pr);
+        } else {
+          hr += static_cast<T>(e * static_cast<double>(psi.x));
+          hi += static_cast<T>(e * static_cast<double>(psi.y));
+        }
       }
       T2 w;
       w.x = sr * hr - si * hi;
       w.y = sr * hi + si * hr;
-      const uint64_t x = gidx(l);
       if (a.accumulate) {
         const T2 o = out[x];
         w.x += o.x;
@@ -331,13 +369,13 @@ __device__ __forceinline__ void pass_tile(const PassArgs &a, uint64_t g, const d
   }
 }
 
-template <typename T, bool CPLX, bool DOTS>
+template <typename T, bool CPLX, bool DOTS, bool NOISE = false>
 __global__ __launch_bounds__(kThreads) void hpsi_pass_kernel(const PassArgs a) {
   const int t = a.c + a.m;
   const uint64_t ntiles = 1ull << (a.n - t);
   uint64_t g = blockIdx.x;
   if (a.xcd && ntiles >= 8) g = (g & 7) * (ntiles >> 3) + (g >> 3);
-  pass_tile<T, CPLX, DOTS>(a, g, a.diag ? a.dtab + g * kDiagRow : nullptr, blockIdx.x);
+  pass_tile<T, CPLX, DOTS, NOISE>(a, g, a.diag ? a.dtab + g * kDiagRow : nullptr, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -767,6 +805,79 @@ __global__ __launch_bounds__(kThreads) void scale_kernel(uint64_t npk, void *vv,
   }
 }
 
+// One launch of the Arnoldi modified Gram-Schmidt chain: w <- w - h u_prev
+// with h = pscale2 * (rec[1], rec[2]) (the previous launch's reduced
+// overlap, read from device memory), then the partial sums ||w||^2 and
+// <u_next|w> of the updated w.  u_prev / u_next may be null.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void arnoldi_mgs_kernel(uint64_t npk, void *wv,
+                                                               const void *upv, const double *rec,
+                                                               double pscale2, const void *unv,
+                                                               double *partials) {
+  using P = typename Packet<T>::type;
+  P *w = reinterpret_cast<P *>(wv);
+  const P *up = reinterpret_cast<const P *>(upv);
+  const P *un = reinterpret_cast<const P *>(unv);
+  double hr = 0.0, hi = 0.0;
+  if (up) {
+    hr = pscale2 * rec[1];
+    hi = pscale2 * rec[2];
+  }
+  double acc[3] = {0.0, 0.0, 0.0};
+  __shared__ double red[kWaves * 3];
+  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  for (uint64_t x = (uint64_t)blockIdx.x * kThreads + threadIdx.x; x < npk; x += stride) {
+    P z = w[x];
+    P a, b;
+    if (up) a = up[x];
+    if (un) b = un[x];
+#pragma unroll
+    for (int k = 0; k < Packet<T>::amps; ++k) {
+      if (up) {
+        const double ar = packet_re(a, k), ai = packet_im(a, k);
+        const double zr = packet_re(z, k) - (hr * ar - hi * ai);
+        const double zi = packet_im(z, k) - (hr * ai + hi * ar);
+        packet_set(z, k, zr, zi);
+      }
+      const double qr = packet_re(z, k), qi = packet_im(z, k);
+      acc[0] += qr * qr + qi * qi;
+      if (un) {
+        const double br = packet_re(b, k), bi = packet_im(b, k);
+        acc[1] += br * qr + bi * qi;
+        acc[2] += br * qi - bi * qr;
+      }
+    }
+    if (up) w[x] = z;
+  }
+  block_reduce_store<3>(acc, red, partials + (uint64_t)blockIdx.x * kNumDots);
+}
+
+// A 2x2 operator on one bit: thread p owns the pair (x0, x0 | 2^bit).
+struct Op1q {
+  double a[8];  // A00, A01, A10, A11 as (re, im)
+};
+template <typename T>
+__global__ __launch_bounds__(kThreads) void apply_1q_kernel(uint64_t npairs, void *psiv, int bit,
+                                                            const Op1q op) {
+  using T2 = typename Vec2<T>::type;
+  T2 *psi = reinterpret_cast<T2 *>(psiv);
+  const uint64_t lomask = (1ull << bit) - 1;
+  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  for (uint64_t p = (uint64_t)blockIdx.x * kThreads + threadIdx.x; p < npairs; p += stride) {
+    const uint64_t x0 = ((p & ~lomask) << 1) | (p & lomask), x1 = x0 | (1ull << bit);
+    const T2 u = psi[x0], v = psi[x1];
+    const double ur = u.x, ui = u.y, vr = v.x, vi = v.y;
+    const double *A = op.a;
+    T2 o0, o1;
+    o0.x = static_cast<T>(A[0] * ur - A[1] * ui + A[2] * vr - A[3] * vi);
+    o0.y = static_cast<T>(A[0] * ui + A[1] * ur + A[2] * vi + A[3] * vr);
+    o1.x = static_cast<T>(A[4] * ur - A[5] * ui + A[6] * vr - A[7] * vi);
+    o1.y = static_cast<T>(A[4] * ui + A[5] * ur + A[6] * vi + A[7] * vr);
+    psi[x0] = o0;
+    psi[x1] = o1;
+  }
+}
+
 template <typename S, typename D2>
 __global__ __launch_bounds__(kThreads) void convert_kernel(uint64_t D,
                                                            const typename Vec2<S>::type *src,
@@ -1106,7 +1217,7 @@ void default_tiles(int dtype, int kind, int &tl, int &cmin) {
   if (cmin <= 0) cmin = dtype == EMU_SV_C64 ? kDefaultContigC64 : kDefaultContigC128;
 }
 
-template <typename T, bool CPLX, bool DOTS>
+template <typename T, bool CPLX, bool DOTS, bool NOISE>
 int set_lds_attr(size_t lds) {
   // Dynamic LDS above 64 KiB must be enabled per kernel.
   static std::mutex mu;
@@ -1114,21 +1225,21 @@ int set_lds_attr(size_t lds) {
   std::lock_guard<std::mutex> lock(mu);
   if (lds <= enabled) return EMU_SV_OK;
   hipError_t err =
-      hipFuncSetAttribute(reinterpret_cast<const void *>(&hpsi_pass_kernel<T, CPLX, DOTS>),
+      hipFuncSetAttribute(reinterpret_cast<const void *>(&hpsi_pass_kernel<T, CPLX, DOTS, NOISE>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (err == hipSuccess) enabled = lds;
   return hip_check(err, "hipFuncSetAttribute(MaxDynamicSharedMemorySize)");
 }
 
-template <typename T, bool CPLX, bool DOTS>
+template <typename T, bool CPLX, bool DOTS, bool NOISE = false>
 int launch_pass(const PassArgs &a, hipStream_t st) {
   const int t = a.c + a.m;
   const uint64_t nt = 1ull << (a.n - t);
   const size_t lds = sizeof(typename Vec2<T>::type) << t;
-  int rc = set_lds_attr<T, CPLX, DOTS>(lds);
+  int rc = set_lds_attr<T, CPLX, DOTS, NOISE>(lds);
   if (rc) return rc;
-  hipLaunchKernelGGL((hpsi_pass_kernel<T, CPLX, DOTS>), dim3((unsigned)nt), dim3(kThreads), lds,
-                     st, a);
+  hipLaunchKernelGGL((hpsi_pass_kernel<T, CPLX, DOTS, NOISE>), dim3((unsigned)nt), dim3(kThreads),
+                     lds, st, a);
   return hip_check(hipGetLastError(), "hpsi_pass_kernel launch");
 }
 
@@ -1163,8 +1274,10 @@ struct OrthoIn {
 template <typename T>
 int run_hpsi(int n, const double *ham, double sr, double si, const void *v, double vsc,
              const void *prev, double psc, void *w, double *dots, void *ws, int kind, int tl,
-             int cmin, bool cplx, hipStream_t st, const OrthoIn *ortho = nullptr) {
+             int cmin, bool cplx, hipStream_t st, const OrthoIn *ortho = nullptr,
+             bool noise = false) {
   if (kind == kKernelWave && n < tl) kind = kKernelLds;  // one small tile: LDS kernel
+  if (noise && kind != kKernelLds) return fail(EMU_SV_EINVAL, "hpsi: the noise term needs the LDS kernel");
   Pass plan[EMU_SV_MAX_QUBITS];
   const int np = make_plan(n, tl, cmin, plan);
   {
@@ -1208,7 +1321,9 @@ int run_hpsi(int n, const double *ham, double sr, double si, const void *v, doub
     const bool last = p == np - 1;
     const bool fuse = last && dots != nullptr;
     int rc;
-    if (kind == kKernelWave) {
+    if (noise) {
+      rc = fuse ? launch_pass<T, true, true, true>(a, st) : launch_pass<T, true, false, true>(a, st);
+    } else if (kind == kKernelWave) {
       switch (tl - 6) {
         case 3: rc = launch_wave_any<T, 3>(a, cplx, fuse, st); break;
         case 4: rc = launch_wave_any<T, 4>(a, cplx, fuse, st); break;
@@ -1420,6 +1535,8 @@ struct BatchLayout {
 int batch_plan(const emu_sv_batch_register &r, int dtype, int tl, int cmin, BatchReg &out) {
   if (r.n < 1 || r.n > 30 || r.steps < 1 || !r.ham || !r.dt || !r.psi || !r.psi_alt || !r.rows)
     return fail(EMU_SV_EINVAL, "batch: bad register description");
+  if (r.flags & EMU_SV_FLAG_NOISE)
+    return fail(EMU_SV_EINVAL, "batch: the Lanczos engine needs a Hermitian H (no noise term)");
   if ((reinterpret_cast<uintptr_t>(r.psi) | reinterpret_cast<uintptr_t>(r.psi_alt) |
        reinterpret_cast<uintptr_t>(r.rows)) & 15)
     return fail(EMU_SV_EINVAL, "batch: psi, psi_alt and rows must be 16-byte aligned");
@@ -1626,6 +1743,16 @@ int emu_sv_pack_hamiltonian(int n, const double *omega, const double *delta, con
   return EMU_SV_OK;
 }
 
+int emu_sv_pack_noise(const double *noise, double *block) {
+  if (!noise || !block) return fail(EMU_SV_EINVAL, "pack_noise: null pointer");
+  for (int k = 0; k < 8; ++k) {
+    if (!std::isfinite(noise[k])) return fail(EMU_SV_EINVAL, "pack_noise: non-finite entry");
+    block[kNoise + k] = noise[k];
+  }
+  block[kFlagNoise] = 1.0;
+  return EMU_SV_OK;
+}
+
 int emu_sv_plan_passes(int n, int dtype, int tile_log, int min_contig, int flags) {
   if (int rc = check_n(n)) return rc;
   if (int rc = check_dtype(dtype)) return rc;
@@ -1664,7 +1791,13 @@ int emu_sv_hpsi(int n, int dtype, const double *ham, double sigma_re, double sig
   int kind = kernel_kind(flags);
   if (kind != kKernelLds && kind != kKernelWave && kind != kKernelTile)
     return fail(EMU_SV_EINVAL, "hpsi: unknown kernel kind in flags");
-  const bool cplx = (flags & EMU_SV_FLAG_COMPLEX) != 0;
+  const bool noise = (flags & EMU_SV_FLAG_NOISE) != 0;
+  const bool cplx = noise || (flags & EMU_SV_FLAG_COMPLEX) != 0;
+  if (noise && kind != kKernelLds) {  // only the LDS kernel carries the noise term
+    kind = kKernelLds;
+    tile_log = 0;
+    min_contig = 0;
+  }
   int tla = kind == kKernelTile ? tile_first_log(flags, 0) : 0;
   if (kind == kKernelTile) {
     int rc = check_tile_schedule(n, dtype, tile_log, min_contig, tla, "hpsi");
@@ -1695,9 +1828,9 @@ int emu_sv_hpsi(int n, int dtype, const double *ham, double sigma_re, double sig
     return fail(EMU_SV_EINVAL, "hpsi: min_contig must be in [0, tile_log)");
   if (dtype == EMU_SV_C128)
     return run_hpsi<double>(n, ham, sigma_re, sigma_im, v, v_scale, prev, prev_scale, w, dots_out,
-                            workspace, kind, tile_log, min_contig, cplx, stream);
+                            workspace, kind, tile_log, min_contig, cplx, stream, nullptr, noise);
   return run_hpsi<float>(n, ham, sigma_re, sigma_im, v, v_scale, prev, prev_scale, w, dots_out,
-                         workspace, kind, tile_log, min_contig, cplx, stream);
+                         workspace, kind, tile_log, min_contig, cplx, stream, nullptr, noise);
 }
 
 int emu_sv_lanczos_matvec(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
@@ -1714,6 +1847,8 @@ int emu_sv_lanczos_matvec(int n, int dtype, const double *ham, double sigma_re, 
   int kind = kernel_kind(flags);
   if (kind != kKernelLds && kind != kKernelWave && kind != kKernelTile)
     return fail(EMU_SV_EINVAL, "lanczos_matvec: unknown kernel kind in flags");
+  if (flags & EMU_SV_FLAG_NOISE)
+    return fail(EMU_SV_EINVAL, "lanczos_matvec: the Lanczos recurrence needs a Hermitian H (no noise term)");
   bool use_tile = false;
   int tla = kind == kKernelTile ? tile_first_log(flags, 0) : 0;
   if (kind == kKernelTile) {
@@ -1772,6 +1907,8 @@ int emu_sv_lanczos_matvec_dev(int n, int dtype, const double *ham, double sigma_
     return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: w, r, pa and pb must be distinct buffers");
   if (kernel_kind(flags) != kKernelTile)
     return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: needs the tile kernel kind");
+  if (flags & EMU_SV_FLAG_NOISE)
+    return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: the Lanczos recurrence needs a Hermitian H (no noise term)");
   int tla = tile_first_log(flags, 0);
   const int rc = check_tile_schedule(n, dtype, tile_log, min_contig, tla, "lanczos_matvec_dev");
   if (rc < 0) return rc;
@@ -1869,6 +2006,57 @@ int emu_sv_dot(int n, int dtype, const void *a, const void *b, double *dots_out,
     hipLaunchKernelGGL(dot_kernel<float>, dim3(nb), dim3(kThreads), 0, stream, npk, a, b, part);
   if (int rc = hip_check(hipGetLastError(), "dot launch")) return rc;
   return finalize(part, nb, 3, workspace, n, dots_out, stream);
+}
+
+int emu_sv_arnoldi_orthogonalize(int n, int dtype, int count, const void *const *vecs,
+                                 const double *scales, void *w, double *records, void *workspace,
+                                 hipStream_t stream) {
+  if (int rc = check_n(n)) return rc;
+  if (int rc = check_dtype(dtype)) return rc;
+  if (count < 1 || count > kMaxLincomb)
+    return fail(EMU_SV_EINVAL, "arnoldi_orthogonalize: count must be in [1, 1024]");
+  if (!vecs || !scales || !w || !records || !workspace)
+    return fail(EMU_SV_EINVAL, "arnoldi_orthogonalize: null pointer");
+  for (int k = 0; k < count; ++k) {
+    if (!vecs[k]) return fail(EMU_SV_EINVAL, "arnoldi_orthogonalize: null basis vector");
+    if (vecs[k] == w) return fail(EMU_SV_EINVAL, "arnoldi_orthogonalize: w aliases a basis vector");
+  }
+  const uint64_t npk = packets(n, dtype);
+  const int nb = stream_blocks(npk);
+  double *part = partials_ptr(workspace, n);
+  for (int k = 0; k <= count; ++k) {
+    const void *up = k > 0 ? vecs[k - 1] : nullptr;
+    const void *un = k < count ? vecs[k] : nullptr;
+    const double *rec = k > 0 ? records + 4 * (k - 1) : records;
+    const double ps2 = k > 0 ? scales[k - 1] * scales[k - 1] : 0.0;
+    if (dtype == EMU_SV_C128)
+      hipLaunchKernelGGL(arnoldi_mgs_kernel<double>, dim3(nb), dim3(kThreads), 0, stream, npk, w,
+                         up, rec, ps2, un, part);
+    else
+      hipLaunchKernelGGL(arnoldi_mgs_kernel<float>, dim3(nb), dim3(kThreads), 0, stream, npk, w,
+                         up, rec, ps2, un, part);
+    if (int rc = hip_check(hipGetLastError(), "arnoldi_mgs launch")) return rc;
+    if (int rc = finalize(part, nb, 3, workspace, n, records + 4 * k, stream)) return rc;
+  }
+  return EMU_SV_OK;
+}
+
+int emu_sv_apply_1q(int n, int dtype, void *psi, int bit, const double *a, hipStream_t stream) {
+  if (int rc = check_n(n)) return rc;
+  if (int rc = check_dtype(dtype)) return rc;
+  if (!psi || !a) return fail(EMU_SV_EINVAL, "apply_1q: null pointer");
+  if (bit < 0 || bit >= n) return fail(EMU_SV_EINVAL, "apply_1q: bit must be in [0, n)");
+  Op1q op;
+  for (int k = 0; k < 8; ++k) op.a[k] = a[k];
+  const uint64_t npairs = 1ull << (n - 1);
+  const int nb = stream_blocks(npairs);
+  if (dtype == EMU_SV_C128)
+    hipLaunchKernelGGL(apply_1q_kernel<double>, dim3(nb), dim3(kThreads), 0, stream, npairs, psi,
+                       bit, op);
+  else
+    hipLaunchKernelGGL(apply_1q_kernel<float>, dim3(nb), dim3(kThreads), 0, stream, npairs, psi,
+                       bit, op);
+  return hip_check(hipGetLastError(), "apply_1q launch");
 }
 
 int emu_sv_scale(int n, int dtype, void *v, double s, hipStream_t stream) {

@@ -26,7 +26,8 @@ from emulators_amd.emu_base import SequenceData
 from emulators_amd.emu_sv.observables import Results, Statistics
 from emulators_amd.emu_sv.state_vector import StateVector
 from emulators_amd.emu_sv.sv_config import SVConfig
-from emulators_amd.emu_sv.time_evolution import EvolveStateVector
+from emulators_amd.emu_sv.sv_config import Solver
+from emulators_amd.emu_sv.time_evolution import EvolveMonteCarlo, EvolveStateVector
 from emulators_amd.emu_sv.dense_state import DenseState
 
 _TIME_CONVERSION_COEFF = 0.001  # Omega and delta in rad/us, dt in ns
@@ -34,14 +35,23 @@ _TIME_CONVERSION_COEFF = 0.001  # Omega and delta in rad/us, dt in ns
 
 class SVBackendImpl:
     def __init__(self, config: SVConfig, data: SequenceData):
-        if data.lindblad_ops:
-            raise NotImplementedError(
-                "Lindblad noise (density-matrix or Monte-Carlo solvers) is not part of the GPU "
-                "state-vector path"
-            )
+        # stepper choice of sv_backend_impl.py:81-95: Lindblad noise runs as
+        # Monte-Carlo trajectories when the noise is stochastic (or asked for);
+        # the density-matrix solver is not part of this package
+        self.pulser_lindblads = list(data.lindblad_ops)
+        if self.pulser_lindblads:
+            solver = getattr(config, "solver", Solver.DEFAULT)
+            if (solver == Solver.DEFAULT and data.has_stochastic_noise) or \
+                    solver == Solver.MONTECARLO:
+                self.stepper = EvolveMonteCarlo(config.max_krylov_dim)
+            else:
+                raise NotImplementedError(
+                    "the Lindblad density-matrix solver is not part of the GPU state-vector path; "
+                    "use solver=Solver.MONTECARLO")
+        else:
+            self.stepper = EvolveStateVector(config.max_krylov_dim)
         self._config = config
         self._data = data
-        self.stepper = EvolveStateVector(config.max_krylov_dim)
         self.target_times = data.target_times
         self.omega = data.omega
         self.delta = data.delta
@@ -129,7 +139,7 @@ class SVBackendImpl:
         out, self._current_H = self.stepper.apply(
             dt * _TIME_CONVERSION_COEFF, self.omega[step_idx], self.delta[step_idx],
             self.phi[step_idx], self.interaction_matrix(self.target_times[step_idx])[0],
-            self._work, self._config.krylov_tolerance, [])
+            self._work, self._config.krylov_tolerance, self.pulser_lindblads)
         self._work = out
 
     def _callbacks_at(self, norm_time: float) -> list:
@@ -150,7 +160,8 @@ class SVBackendImpl:
         if self._current_H is None:
             nxt = min(step_idx + 1, len(self.target_times) - 1)
             self._current_H = self.stepper.get_hamiltonian(
-                omegas=self.omega[0], deltas=self.delta[0], phis=self.phi[0], pulser_lindblads=[],
+                omegas=self.omega[0], deltas=self.delta[0], phis=self.phi[0],
+                pulser_lindblads=self.pulser_lindblads,
                 interaction_matrix=self.interaction_matrix(
                     0.5 * (self.target_times[step_idx] + self.target_times[nxt]))[0],
                 device=state.data.device)

@@ -1,5 +1,6 @@
 from .interface import BaseStepper
-from .implementations import EvolveStateVector
+from .implementations import EvolveMonteCarlo, EvolveStateVector
 from .torch import DifferentiableStateVectorEvolution
 
-__all__ = ["BaseStepper", "DifferentiableStateVectorEvolution", "EvolveStateVector"]
+__all__ = ["BaseStepper", "DifferentiableStateVectorEvolution", "EvolveMonteCarlo",
+           "EvolveStateVector"]

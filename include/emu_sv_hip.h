@@ -41,8 +41,12 @@ extern "C" {
 #define EMU_SV_C128 0
 #define EMU_SV_C64 1
 
-/* emu_sv_hpsi flags: bit 0 complex coefficients, bits 8-11 kernel kind */
+/* emu_sv_hpsi flags: bit 0 complex coefficients, bit 1 noise, bits 8-11
+ * kernel kind */
 #define EMU_SV_FLAG_COMPLEX 1 /* some phi != 0: complex flip coefficients */
+#define EMU_SV_FLAG_NOISE 2   /* the block carries the non-Hermitian term   */
+                              /* -i/2 sum L^+L (emu_sv_pack_noise); implies */
+                              /* complex coefficients and the LDS kernel    */
 #define EMU_SV_KERNEL_DEFAULT (0 << 8) /* library choice (the wave kernel)  */
 #define EMU_SV_KERNEL_LDS (1 << 8)     /* workgroup tiles staged in LDS     */
 #define EMU_SV_KERNEL_WAVE (2 << 8)    /* wavefront tiles in registers,     */
@@ -66,8 +70,14 @@ extern "C" {
  *   [96,96+1024)  symmetric U_b[b][b'] (upper triangle of the qubit matrix
  *                 mirrored, zero diagonal)
  *   [1120]        flag: 1.0 if any Im c_b != 0 (the reference's complex
- *                 sigma path, emu_sv/hamiltonian.py:61)            */
-#define EMU_SV_HAM_DOUBLES 1128
+ *                 sigma path, emu_sv/hamiltonian.py:61)
+ *   [1121]        flag: 1.0 if the noise term below is present
+ *   [1122,1130)   noise 2x2 complex K = -i/2 sum_l L_l^+ L_l, row-major
+ *                 (re, im) pairs K00, K01, K10, K11 in the (g, r) basis;
+ *                 applied to every qubit: <1|H|0> += K10, <0|H|1> += K01,
+ *                 diagonal += K00 per g bit and K11 per r bit
+ *                 (emu_sv/hamiltonian.py:61-67, 147-151)             */
+#define EMU_SV_HAM_DOUBLES 1136
 
 /* Last error message of the calling thread ("" if none). */
 const char *emu_sv_last_error(void);
@@ -83,6 +93,14 @@ const char *emu_sv_version(void);
 int emu_sv_pack_hamiltonian(int n, const double *omega, const double *delta,
                             const double *phi, const double *interaction,
                             double *block_out);
+
+/* Add the single-qubit noise term K = -i/2 sum_l L_l^+ L_l (2x2 complex,
+ * 8 doubles row-major (re, im), host) to a packed block (host) and set its
+ * noise flag.  Replaces the noise part of RydbergHamiltonian.__init__ and
+ * _create_diagonal (emu_sv/hamiltonian.py:61-67, 147-151); K itself is
+ * compute_noise_from_lindbladians (emu_base/jump_lindblad_operators.py:80-93).
+ * Kernels given EMU_SV_FLAG_NOISE apply it on the fly. */
+int emu_sv_pack_noise(const double *noise, double *block_inout);
 
 /* Number of H.psi passes the planner uses for n qubits, element type and
  * schedule (each pass streams the vector once). */
@@ -183,6 +201,28 @@ int emu_sv_dot(int n, int dtype, const void *a, const void *b, double *dots_out,
 
 /* v <- s * v (real s). */
 int emu_sv_scale(int n, int dtype, void *v, double s, hipStream_t stream);
+
+/* Full modified Gram-Schmidt of w against count basis vectors V_k (stored
+ * unnormalised; the orthonormal vector is scales[k] * V_k), in the order of
+ * the Arnoldi branch of krylov_exp.py:52-58: for k = 0..count-1,
+ * h_k = <v_k|w>, w <- w - h_k v_k.  The coefficients never leave the device:
+ * launch k subtracts the previous projection (read from records_out) and
+ * reduces the next overlap in the same pass over w.  records_out (device,
+ * 4*(count+1) doubles): record k < count = {||w||^2 before the k-th
+ * subtraction, Re, Im <V_k|w>} (unscaled: h_k = scales[k] * <V_k|w>), record
+ * count = {||w||^2 after the last subtraction}.  vecs: host array of device
+ * pointers; scales: host doubles; count in [1, 1024]. */
+int emu_sv_arnoldi_orthogonalize(int n, int dtype, int count, const void *const *vecs,
+                                 const double *scales, void *w, double *records_out,
+                                 void *workspace, hipStream_t stream);
+
+/* psi <- (A acting on qubit bit b) psi, A a 2x2 complex matrix (8 doubles
+ * row-major (re, im), host) in the (bit 0, bit 1) = (g, r) basis: for each
+ * pair (x, x | 2^b) with bit b clear, (a0, a1) <- (A00 a0 + A01 a1,
+ * A10 a0 + A11 a1).  Replaces emu_sv/algebra.py:28-37 (the quantum-jump
+ * operator, qubit q = bit n-1-q). */
+int emu_sv_apply_1q(int n, int dtype, void *psi, int bit, const double *a,
+                    hipStream_t stream);
 
 /* Copy with element-type conversion (c128 <-> c64 or same type). */
 int emu_sv_convert(int n, int src_dtype, const void *src, int dst_dtype,

@@ -24,12 +24,20 @@ How each part is pinned (tests/test_oracle_*.py):
   * time loop: single-atom Rabi closed form, the reference's SequenceData
     pi-pulse table (occupation 0 -> 1), norm and energy conservation.
   * PCHIP: the reference's PCHIP formula tests restated as closed forms.
+  * noise term: dense brute force H + sum_q K_q, and the reference's check
+    that K = diag(0, -i/2) leaves Re<psi|H|psi> unchanged.
+  * Brent root finder: the reference's known answers (roots and exact
+    evaluation counts 12, 13, 18) and a dense-expm jump-time root.
+  * expect_batch / apply_1q: dense Kronecker-product operators.
+  * Monte-Carlo step: a single decaying atom, whose jump time has the closed
+    form exp(-gamma t) = threshold.
 """
 
 from __future__ import annotations
 
 import cmath
 import math
+import random
 from dataclasses import dataclass, field
 from typing import Callable, Sequence
 
@@ -44,31 +52,42 @@ TIME_CONVERSION_COEFF = 0.001  # emu_sv/sv_backend_impl.py:29 (rad/us with dt in
 # Rydberg Hamiltonian  (emu_sv/hamiltonian.py)
 # ---------------------------------------------------------------------------
 class OracleHamiltonian:
-    """Restates ``RydbergHamiltonian`` (emu_sv/hamiltonian.py:5-172), noise-free.
+    """Restates ``RydbergHamiltonian`` (emu_sv/hamiltonian.py:5-172).
 
     H = sum_j (O_j/2)[cos(phi_j) X_j + sin(phi_j) Y_j] - sum_j d_j n_j
-        + sum_{i<j} U_ij n_i n_j,  qubit 0 = most significant bit.
+        + sum_{i<j} U_ij n_i n_j,  qubit 0 = most significant bit,
+    plus, with ``noise`` = K (2x2), K on every qubit (hamiltonian.py:61-67,
+    147-151).
     """
 
-    def __init__(self, omegas, deltas, phis, interaction_matrix):
+    def __init__(self, omegas, deltas, phis, interaction_matrix, noise=None):
         # hamiltonian.py:51-67
         self.nqubits = len(omegas)
         self.omegas = torch.as_tensor(omegas).detach().to(DTYPE) / 2.0
         self.deltas = torch.as_tensor(deltas).detach().to(DTYPE)
         self.phis = torch.as_tensor(phis).detach().to(DTYPE)
         self.interaction_matrix = torch.as_tensor(interaction_matrix).detach()
+        self.noise = None if noise is None else torch.as_tensor(noise, dtype=DTYPE)
         self.diag = self._create_diagonal()
         self.inds = torch.tensor([1, 0])
-        self.complex = bool(self.phis.any())
+        self.complex = bool(self.phis.any()) or self.noise is not None
         if self.complex:
             oc = self.omegas * torch.exp(1.0j * self.phis)
             self.omegas_c = torch.stack((oc, oc.conj()), dim=1)
+            if self.noise is not None:
+                self.omegas_c[:, 0] += self.noise[1, 0]
+                self.omegas_c[:, 1] += self.noise[0, 1]
 
     def _create_diagonal(self) -> torch.Tensor:
         # hamiltonian.py:139-162: -sum_i d_i n_i + sum_{i<j} U_ij n_i n_j,
-        # upper triangle of U only.
+        # upper triangle of U only; the noise diagonal first (:147-151).
         n = self.nqubits
         diag = torch.zeros(2**n, dtype=DTYPE)
+        if self.noise is not None:
+            for i in range(n):
+                diag = diag.view(2**i, 2, -1)
+                diag[:, 0, :] += self.noise[0, 0]
+                diag[:, 1, :] += self.noise[1, 1]
         for i in range(n):
             diag = diag.view(2**i, 2, -1)
             i_fixed = diag[:, 1, :]
@@ -100,6 +119,15 @@ class OracleHamiltonian:
         return torch.vdot(state, self * state).real
 
 
+def noise_from_lindbladians(lindbladians) -> torch.Tensor:
+    """K = -i/2 sum_l L_l^+ L_l (emu_base/jump_lindblad_operators.py:80-93)."""
+    acc = torch.zeros(2, 2, dtype=DTYPE)
+    for L in lindbladians:
+        L = torch.as_tensor(L, dtype=DTYPE)
+        acc = acc + L.conj().T @ L
+    return -0.5j * acc
+
+
 # ---------------------------------------------------------------------------
 # Dense brute force  (test/utils_testing/utils_dense_hamiltonians.py:8-67)
 # ---------------------------------------------------------------------------
@@ -128,6 +156,17 @@ def dense_rydberg_hamiltonian(omegas, deltas, phis, interaction_matrix) -> torch
         h -= de[i] * _single_gate(i, n, nop)
         for j in range(i + 1, n):
             h += um[i, j] * (_single_gate(i, n, nop) @ _single_gate(j, n, nop))
+    return h
+
+
+def dense_noisy_hamiltonian(omegas, deltas, phis, interaction_matrix, noise) -> torch.Tensor:
+    """Kronecker-product H + sum_q K_q: the brute-force form of the noisy
+    Hamiltonian of hamiltonian.py:61-67, 147-151 (K on every qubit)."""
+    n = len(omegas)
+    h = dense_rydberg_hamiltonian(omegas, deltas, phis, interaction_matrix)
+    k = torch.as_tensor(noise, dtype=DTYPE)
+    for q in range(n):
+        h = h + _single_gate(q, n, k)
     return h
 
 
@@ -547,3 +586,141 @@ def midpoint_samples(samples_1ns: torch.Tensor, target_times: Sequence[float],
     if is_amplitude:
         out[-1] = torch.where(out[-1] > 0, out[-1], torch.zeros_like(out[-1]))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Quantum jumps  (emu_base/math/brents_root_finding.py, emu_sv/algebra.py,
+# emu_sv/time_evolution/implementations.py:123-250)
+# ---------------------------------------------------------------------------
+class OracleBrent:
+    """Brent's bracketing root finder, driven one abscissa at a time
+    (brents_root_finding.py:5-110): inverse quadratic interpolation through
+    (a, b, c), secant when two ordinates nearly coincide, bisection when the
+    step leaves the bracket or does not halve."""
+
+    def __init__(self, start, end, f_start, f_end, epsilon=1e-6):
+        assert start <= end and f_start * f_end < 0
+        self.eps = epsilon
+        self.a, self.b, self.fa, self.fb = start, end, f_start, f_end
+        if abs(self.fa) < abs(self.fb):  # b is the better end (:32-35)
+            self.a, self.b, self.fa, self.fb = self.b, self.a, self.fb, self.fa
+        self.c, self.d, self.fc = self.a, self.a, self.fa
+        self.bisect = True
+        self.pending = None
+
+    def next_abscissa(self) -> float:
+        # :41-78
+        a, b, c, d, fa, fb, fc = self.a, self.b, self.c, self.d, self.fa, self.fb, self.fc
+        if abs(fc - fa) < self.eps or abs(fc - fb) < self.eps:
+            dx = fb * (b - a) / (fa - fb)
+        else:
+            s_, r_, t_ = fb / fa, fb / fc, fa / fc
+            dx = s_ * (t_ * (r_ - t_) * (c - b) + (r_ - 1) * (b - a)) / ((t_ - 1) * (s_ - 1) * (r_ - 1))
+        ref = abs(b - c) if self.bisect else abs(c - d)
+        if (abs(dx) >= abs(0.75 * (a - b)) or dx * (a - b) < 0 or abs(dx) >= ref / 2
+                or ref < abs(2 * self.eps * b)):
+            dx, self.bisect = (a - b) / 2, True
+        else:
+            self.bisect = False
+        self.pending = b + dx
+        self.d, self.c, self.fc = c, b, fb
+        return self.pending
+
+    def provide(self, x: float, fx: float) -> None:
+        # :80-97
+        assert x == self.pending
+        if self.fa * fx < 0:
+            self.b, self.fb = x, fx
+        else:
+            self.a, self.fa = x, fx
+        if abs(self.fa) < abs(self.fb):
+            self.a, self.b, self.fa, self.fb = self.b, self.a, self.fb, self.fa
+
+    def converged(self, tol: float) -> bool:
+        return abs(self.b - self.a) < tol
+
+
+def brent_root(f, start, end, tolerance=1e-6, epsilon=1e-6):
+    """find_root_brents (brents_root_finding.py:113-150); returns (root, calls)."""
+    calls = [0]
+
+    def g(x):
+        calls[0] += 1
+        return f(x)
+
+    r = OracleBrent(start, end, g(start), g(end), epsilon)
+    while not r.converged(tolerance):
+        x = r.next_abscissa()
+        r.provide(x, g(x))
+    return r.b, calls[0]
+
+
+def expect_batch(state: torch.Tensor, ops: torch.Tensor, n: int) -> torch.Tensor:
+    """T[q, i] = Tr(ops_i rho_q) (emu_sv/algebra.py:4-25)."""
+    rows = []
+    for q in range(n):
+        v = state.view(2**q, 2, -1)
+        rho = torch.tensordot(v, v.conj(), dims=([0, 2], [0, 2]))
+        rows.append(torch.stack([torch.trace(op @ rho) for op in ops]))
+    return torch.stack(rows)
+
+
+def apply_1q(state: torch.Tensor, q: int, op: torch.Tensor) -> torch.Tensor:
+    """op on qubit q (emu_sv/algebra.py:28-37)."""
+    return (torch.as_tensor(op, dtype=DTYPE) @ state.view(2**q, 2, -1)).reshape(-1)
+
+
+class OracleMonteCarlo:
+    """EvolveMonteCarlo (implementations.py:123-250) with the CPU Hamiltonian
+    and the oracle's Arnoldi krylov_exp; draws from Python's ``random`` in
+    the same order (one threshold at construction, random.choices and a new
+    threshold per jump)."""
+
+    JUMP_TIME_TOLERANCE_FRACTION = 0.1  # implementations.py:13
+
+    def __init__(self, max_krylov_dim: int = DEFAULT_MAX_KRYLOV_DIM):
+        self.max_krylov_dim = max_krylov_dim
+        self.jump_threshold = random.uniform(0.0, 1.0)
+        self.jumps: list = []
+
+    def evolve(self, dt, ham, state, tol):
+        return krylov_exp(lambda x: -1j * dt * (ham * x), state, exp_tolerance=tol,
+                          norm_tolerance=tol, is_hermitian=False,
+                          max_krylov_dim=self.max_krylov_dim)
+
+    def jump(self, state, lindblads, aggregated, n):
+        w = expect_batch(state, aggregated, n).real
+        q, op = random.choices([(q, L) for q in range(n) for L in lindblads],
+                               weights=w.reshape(-1).tolist())[0]
+        self.jumps.append(q)
+        state = apply_1q(state, q, op)
+        state = state / state.norm()
+        self.jump_threshold = random.uniform(0.0, state.norm().item() ** 2)
+        return state
+
+    def step(self, dt, omegas, deltas, phis, interaction_matrix, state, tol, lindblads):
+        ham = OracleHamiltonian(omegas, deltas, phis, interaction_matrix,
+                                noise=noise_from_lindbladians(lindblads))
+        n = len(omegas)
+        ftol = dt * self.JUMP_TIME_TOLERANCE_FRACTION
+        stacked = torch.stack([torch.as_tensor(L, dtype=DTYPE) for L in lindblads])
+        aggregated = stacked.conj().transpose(1, 2) @ stacked
+        t = 0.0
+        gap = state.norm().item() ** 2 - self.jump_threshold
+        while abs(dt - t) > 1e-5:
+            old = gap
+            state = self.evolve(dt - t, ham, state.clone(), tol)
+            gap = state.norm().item() ** 2 - self.jump_threshold
+            if gap > 0.0:
+                break
+            r = OracleBrent(t, dt, old, gap, ftol)
+            t = dt
+            while not r.converged(ftol):
+                x = r.next_abscissa()
+                state = self.evolve(x - t, ham, state.clone(), tol)
+                t = x
+                gap = state.norm().item() ** 2 - self.jump_threshold
+                r.provide(t, gap)
+            state = self.jump(state, lindblads, aggregated, n)
+            gap = state.norm().item() ** 2 - self.jump_threshold
+        return state

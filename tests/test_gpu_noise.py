@@ -1,0 +1,226 @@
+"""The Monte-Carlo noisy path through the C ABI against the oracle.
+
+Bars: the noisy H.psi (LDS kernel with the -i/2 sum L^+L term) against the
+dense Kronecker-product H at atol 1e-12 (complex128) / 2e-5 relative
+(complex64); the device Arnoldi krylov_exp with the oracle's iteration
+count and its result within the tolerance of the oracle's; the modified
+Gram-Schmidt records against a CPU restatement; the jump helpers against
+dense operators; seeded quantum-jump trajectories with the same jumps as the
+oracle's and final states within 1e-8.
+"""
+
+import math
+import random
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import emu_sv_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+LINDBLADS = [math.sqrt(0.7) * torch.tensor([[0, 1], [0, 0]], dtype=torch.complex128),
+             math.sqrt(0.3) * torch.tensor([[1, 0], [0, -1]], dtype=torch.complex128),
+             math.sqrt(0.2) * torch.tensor([[0, 1], [0, 0.5]], dtype=torch.complex128)]
+
+
+def _params(n, seed, phase):
+    g = torch.Generator().manual_seed(seed)
+    om = torch.rand(n, generator=g, dtype=torch.float64) * 4
+    de = torch.randn(n, generator=g, dtype=torch.float64)
+    ph = torch.randn(n, generator=g, dtype=torch.float64) if phase else torch.zeros(n, dtype=torch.float64)
+    u = torch.rand(n, n, generator=g, dtype=torch.float64)
+    return om, de, ph, (u + u.T) / 2
+
+
+@pytest.mark.parametrize("n", [2, 5, 9, 12])
+@pytest.mark.parametrize("phase", [False, True])
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_noisy_hpsi_vs_dense(gpu_device, n, phase, dtype):
+    from emulators_amd.emu_sv.hamiltonian import RydbergHamiltonian
+    print(f"noisy hpsi n={n} phase={phase} {dtype}", flush=True)
+    om, de, ph, u = _params(n, 17 + n, phase)
+    k = orc.noise_from_lindbladians(LINDBLADS)
+    h = RydbergHamiltonian(om, de, ph, u, device=gpu_device, noise=k)
+    torch.manual_seed(n)
+    psi = torch.randn(2**n, dtype=torch.complex128)
+    expected = (orc.OracleHamiltonian(om, de, ph, u, noise=k) * psi) if n > 10 else \
+        orc.dense_noisy_hamiltonian(om, de, ph, u, k) @ psi
+    got = h.apply(psi.to(gpu_device, dtype), sigma=1.0).cpu().to(torch.complex128)
+    if dtype == torch.complex128:
+        assert torch.allclose(got, expected, atol=1e-12 * max(1.0, float(expected.abs().max())), rtol=0)
+    else:
+        assert float((got - expected).abs().max()) <= 2e-5 * float(expected.abs().max())
+    # the diagonal, with the diagonal entries of K only
+    if n <= 9:
+        dense = orc.dense_noisy_hamiltonian(torch.zeros(n), de, torch.zeros(n), u,
+                                            torch.diag(torch.diag(k)))
+        assert torch.allclose(h.diag.cpu(), torch.diagonal(dense), atol=1e-12, rtol=0)
+
+
+def test_noisy_expectation_real_part(gpu_device):
+    """K = diag(0, -i/2) leaves Re<psi|H|psi> unchanged
+    (test_hamiltonian.py:134-172)."""
+    from emulators_amd.emu_sv.hamiltonian import RydbergHamiltonian
+    n = 3
+    torch.manual_seed(0)
+    om, de = torch.randn(n, dtype=torch.float64), torch.randn(n, dtype=torch.float64)
+    u = torch.randn(n, n, dtype=torch.float64)
+    u = u + u.T
+    psi = torch.zeros(8, dtype=torch.complex128)
+    psi[0b111] = psi[0b100] = 1 / math.sqrt(2)
+    psi = psi.to(gpu_device)
+    h0 = RydbergHamiltonian(om, de, torch.zeros(n), u, device=gpu_device)
+    h = RydbergHamiltonian(om, de, torch.zeros(n), u, device=gpu_device,
+                           noise=torch.tensor([[0.0, 0.0], [0.0, -0.5j]], dtype=torch.complex128))
+    expected = 0.5 * (-2 * de[0] - de[1] - de[2] + u[0, 1] + u[0, 2] + u[1, 2])
+    assert abs(float(h0.expect(psi)) - float(expected)) < 1e-12
+    assert abs(float(h.expect(psi)) - float(expected)) < 1e-12
+
+
+@pytest.mark.parametrize("n,count", [(4, 1), (8, 5), (13, 9)])
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_arnoldi_records_vs_mgs(gpu_device, n, count, dtype):
+    from emulators_amd import _native as native
+    print(f"arnoldi records n={n} count={count} {dtype}", flush=True)
+    torch.manual_seed(100 + n)
+    basis = [torch.randn(2**n, dtype=torch.complex128) for _ in range(count)]
+    scales = [1.0 / float(b.norm()) for b in basis]
+    w = torch.randn(2**n, dtype=torch.complex128)
+    dev = [b.to(gpu_device, dtype) for b in basis]
+    wd = w.to(gpu_device, dtype)
+    rec = native.arnoldi_orthogonalize(n, dev, scales, wd).cpu().numpy()
+    # CPU restatement of the MGS loop on the same (rounded) inputs
+    bb = [d.cpu().to(torch.complex128) for d in dev]
+    ww = w.to(dtype).to(torch.complex128)
+    tol = 1e-12 if dtype == torch.complex128 else 1e-5
+    for k in range(count):
+        vk = bb[k] * scales[k]
+        assert abs(rec[k, 0] - float(ww.norm() ** 2)) <= tol * float(ww.norm() ** 2)
+        h = complex(torch.vdot(vk, ww))
+        got = scales[k] * complex(rec[k, 1], rec[k, 2])
+        assert abs(got - h) <= tol * max(1.0, float(ww.norm()))
+        ww = ww - h * vk
+    assert abs(rec[count, 0] - float(ww.norm() ** 2)) <= tol * max(1.0, float(w.norm() ** 2))
+    out = wd.cpu().to(torch.complex128)
+    assert float((out - ww).abs().max()) <= tol * max(1.0, float(w.abs().max()))
+
+
+@pytest.mark.parametrize("n,dt,tol", [(4, 0.3, 1e-10), (8, 0.5, 1e-10), (10, 1.0, 1e-8)])
+@pytest.mark.parametrize("phase", [False, True])
+def test_arnoldi_krylov_vs_oracle(gpu_device, n, dt, tol, phase):
+    from emulators_amd.emu_base.math.krylov_exp import (KrylovStats, ScaledHamiltonian,
+                                                         krylov_exp_impl)
+    from emulators_amd.emu_sv.hamiltonian import RydbergHamiltonian
+    print(f"arnoldi krylov n={n} dt={dt} phase={phase}", flush=True)
+    om, de, ph, u = _params(n, 40 + n, phase)
+    k = orc.noise_from_lindbladians(LINDBLADS)
+    torch.manual_seed(n)
+    psi = torch.randn(2**n, dtype=torch.complex128)
+    href = orc.OracleHamiltonian(om, de, ph, u, noise=k)
+    ref = orc.krylov_exp_impl(lambda x: -1j * dt * (href * x), psi.clone(), is_hermitian=False,
+                              exp_tolerance=tol, norm_tolerance=tol)
+    h = RydbergHamiltonian(om, de, ph, u, device=gpu_device, noise=k)
+    got = krylov_exp_impl(ScaledHamiltonian(h, -1j * dt), psi.to(gpu_device), is_hermitian=False,
+                          exp_tolerance=tol, norm_tolerance=tol)
+    assert got.converged and ref.converged
+    assert got.iteration_count == ref.iteration_count
+    assert float((got.result.cpu() - ref.result).abs().max()) < 10 * tol
+    # against the dense exponential too
+    dense = torch.linalg.matrix_exp(-1j * dt * orc.dense_noisy_hamiltonian(om, de, ph, u, k)) @ psi \
+        if n <= 8 else None
+    if dense is not None:
+        assert float((got.result.cpu() - dense).abs().max()) < 10 * tol
+
+
+def test_arnoldi_not_converged_raises(gpu_device):
+    from emulators_amd.emu_base.math.krylov_exp import ScaledHamiltonian, krylov_exp
+    from emulators_amd.emu_sv.hamiltonian import RydbergHamiltonian
+    n = 8
+    om, de, ph, u = _params(n, 5, False)
+    h = RydbergHamiltonian(om * 10, de, ph, u, device=gpu_device,
+                           noise=orc.noise_from_lindbladians(LINDBLADS))
+    psi = torch.randn(2**n, dtype=torch.complex128, device=gpu_device)
+    with pytest.raises(RecursionError):
+        krylov_exp(ScaledHamiltonian(h, -1j * 2.0), psi, 1e-12, 1e-12, is_hermitian=False,
+                   max_krylov_dim=4)
+    with pytest.raises(ValueError):  # the Lanczos engine refuses a non-Hermitian H
+        krylov_exp(ScaledHamiltonian(h, -1j * 0.1), psi, 1e-10, 1e-10, is_hermitian=True)
+
+
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_jump_helpers_vs_dense(gpu_device, dtype):
+    from emulators_amd.emu_sv.algebra import apply, expect_batch
+    n = 6
+    torch.manual_seed(9)
+    psi = torch.randn(2**n, dtype=torch.complex128)
+    ops = torch.stack([L.conj().T @ L for L in LINDBLADS])
+    got = expect_batch(psi.to(gpu_device, dtype), ops, n)
+    ref = orc.expect_batch(psi, ops, n)
+    tol = 1e-11 if dtype == torch.complex128 else 2e-5 * float(psi.norm() ** 2)
+    assert float((got - ref).abs().max()) < tol
+    for q in (0, 3, 5):
+        d = psi.to(gpu_device, dtype).clone()
+        apply(d, q, LINDBLADS[2])
+        exp_ = orc.apply_1q(psi, q, LINDBLADS[2])
+        assert float((d.cpu().to(torch.complex128) - exp_).abs().max()) < (
+            1e-13 if dtype == torch.complex128 else 1e-5)
+
+
+@pytest.mark.usefixtures("restore_rng_state")
+@pytest.mark.parametrize("seed", [3, 7])
+def test_monte_carlo_trajectory_vs_oracle(gpu_device, seed):
+    """Seeded trajectories: the same jump sequence (qubits and times) and the
+    same states as the oracle's quantum-jump stepper."""
+    from emulators_amd.emu_sv.time_evolution import EvolveMonteCarlo
+    n, steps, dt, tol = 4, 12, 0.1, 1e-10
+    print(f"monte carlo seed={seed}", flush=True)
+    om, de, ph, u = _params(n, seed, False)
+    lind = [math.sqrt(3.0) * torch.tensor([[0, 1], [0, 0]], dtype=torch.complex128),
+            math.sqrt(1.0) * torch.tensor([[0, 1], [0, 0.5]], dtype=torch.complex128)]
+    psi0 = torch.zeros(2**n, dtype=torch.complex128)
+    psi0[0] = 1.0
+    random.seed(seed)
+    ref = orc.OracleMonteCarlo()
+    states_ref = []
+    psi = psi0.clone()
+    for _ in range(steps):
+        psi = ref.step(dt, om, de, ph, u, psi, tol, lind)
+        states_ref.append(psi.clone())
+    random.seed(seed)
+    first = random.uniform(0.0, 1.0)
+    random.seed(seed)
+    mc = EvolveMonteCarlo()
+    assert mc.jump_threshold == first  # the construction draw
+    psi = psi0.to(gpu_device)
+    thresholds = []
+    for s in range(steps):
+        psi, _ = mc.apply(dt, om, de, ph, u, psi, tol, lind)
+        thresholds.append(mc.jump_threshold)
+        assert float((psi.cpu() - states_ref[s]).abs().max()) < 1e-8, f"step {s}"
+    assert mc.jump_threshold == pytest.approx(ref.jump_threshold, rel=1e-9)
+    assert len(ref.jumps) >= 1  # the case exercises at least one jump
+
+
+@pytest.mark.usefixtures("restore_rng_state")
+def test_noisy_backend_run(gpu_device):
+    """SVBackend on a SequenceData with stochastic Lindblad noise runs the
+    Monte-Carlo stepper (sv_backend_impl.py:84-90) and reports occupations of
+    the normalised state."""
+    from emulators_amd import workloads
+    from emulators_amd.emu_sv import Occupation, SVConfig
+    from emulators_amd.emu_sv.time_evolution import EvolveMonteCarlo
+    from emulators_amd.emu_sv.sv_backend_impl import SVBackendImpl
+    import dataclasses
+    random.seed(1)
+    cfg = SVConfig(observables=[Occupation(evaluation_times=[1.0])], log_level=40)
+    seq = workloads.chain10(cfg)
+    seq = dataclasses.replace(seq, lindblad_ops=[math.sqrt(0.5) * torch.tensor(
+        [[0, 1], [0, 0]], dtype=torch.complex128)], has_stochastic_noise=True)
+    impl = SVBackendImpl(cfg, seq)
+    assert isinstance(impl.stepper, EvolveMonteCarlo)
+    res = impl._run()
+    occ = torch.as_tensor(res.get_result(cfg.observables[0].tag, 1.0))
+    assert occ.shape == (10,)
+    assert torch.all(occ >= -1e-12) and torch.all(occ <= 1 + 1e-12)

@@ -1,0 +1,215 @@
+"""The Monte-Carlo (quantum-jump) pieces that run on the host, and the
+oracle's restatement of the noisy path.
+
+Pins: the reference's Brent known answers (roots and exact evaluation counts,
+test/emu_base/math/test_brents_root_finding.py), dense Kronecker-product
+brute force for the noisy Hamiltonian and the single-qubit helpers, the
+reference's check that K = diag(0, -i/2) leaves Re<psi|H|psi> unchanged
+(test/emu_sv/test_hamiltonian.py:134-172), and the closed-form jump time of
+a decaying atom.
+"""
+
+import math
+import random
+
+import pytest
+import torch
+
+from oracle import emu_sv_oracle as orc
+
+TEST_TOL = 1e-10
+
+
+def _counted(f):
+    calls = [0]
+
+    def g(x):
+        calls[0] += 1
+        return f(x)
+
+    return g, calls
+
+
+def _stairs():
+    """test_brents_root_finding.py:67-115: a random monotone piecewise-linear
+    function on [0, 1] and its exact root."""
+    random.seed(0xDEADBEEF)
+    n = 100
+    diff = [random.random() for _ in range(n)]
+    w = 1 / n
+    m = sum(diff) * w
+    init = -m / 2
+
+    def f(x):
+        assert 0 <= x <= 1
+        if x >= 1:
+            return m
+        i = int(x * n)
+        return init + sum(diff[:i]) * w + x % w * diff[i]
+
+    acc, i = init, 0
+    while acc + w * diff[i] <= 0:
+        acc += w * diff[i]
+        i += 1
+    return f, i * w - acc / diff[i]
+
+
+CASES = [
+    (lambda x: (x + 3) * (x - 1) ** 2, -4.0, 4 / 3.0, -3.0, 12),
+    (lambda x: -1.0 if x < 0 else -1 + x * 10, -10.0, 10.0, 0.1, 13),
+]
+
+
+@pytest.mark.parametrize("impl", ["product", "oracle"])
+@pytest.mark.parametrize("case", range(3))
+@pytest.mark.usefixtures("restore_rng_state")
+def test_brent_known_answers(impl, case):
+    from emulators_amd.emu_base.math.brents_root_finding import find_root_brents
+    if case < 2:
+        f, a, b, root, count = CASES[case]
+    else:
+        f, root = _stairs()
+        a, b, count = 0.0, 1.0, 18
+    if impl == "product":
+        g, calls = _counted(f)
+        got = find_root_brents(g, start=a, end=b)
+        n = calls[0]
+    else:
+        got, n = orc.brent_root(f, a, b)
+    assert abs(got - root) < TEST_TOL
+    assert n == count
+
+
+@pytest.mark.usefixtures("restore_rng_state")
+def test_brent_last_evaluation_is_the_root():
+    """test_brents_root_finding.py:118-160: the finder is driven step by step
+    on the norm of a non-Hermitian evolution; its last ordinate belongs to the
+    returned root, so the caller's state is the state at the root."""
+    from emulators_amd.emu_base.math.brents_root_finding import BrentsRootFinder
+    n = 10
+    torch.manual_seed(0xDEADBEEF)
+    lind = torch.rand(n, n, dtype=torch.complex128)
+    hami = torch.rand(n, n, dtype=torch.complex128)
+    hami = hami + hami.T.conj()
+    noisy = hami - 0.5j * 0.1 * lind.T.conj() @ lind
+    psi = torch.rand(n, dtype=torch.complex128)
+    psi /= psi.norm()
+
+    def gap(t):
+        return float((torch.linalg.matrix_exp(-1j * t * noisy) @ psi).norm() ** 2) - 0.5
+
+    f = BrentsRootFinder(start=0.0, end=10.0, f_start=gap(0.0), f_end=gap(10.0), epsilon=1e-6)
+    last = None
+    while not f.is_converged(1e-6):
+        x = f.get_next_abscissa()
+        last = x
+        f.provide_ordinate(x, gap(x))
+    assert f.current_guess == last or abs(gap(f.current_guess)) < 1e-6
+    assert abs(gap(f.current_guess)) < 1e-5
+
+
+def _params(n, seed, phase):
+    g = torch.Generator().manual_seed(seed)
+    om = torch.rand(n, generator=g, dtype=torch.float64) * 4
+    de = torch.randn(n, generator=g, dtype=torch.float64)
+    ph = torch.randn(n, generator=g, dtype=torch.float64) if phase else torch.zeros(n, dtype=torch.float64)
+    u = torch.rand(n, n, generator=g, dtype=torch.float64)
+    return om, de, ph, (u + u.T) / 2
+
+
+LINDBLADS = [math.sqrt(0.7) * torch.tensor([[0, 1], [0, 0]], dtype=torch.complex128),
+             math.sqrt(0.3) * torch.tensor([[1, 0], [0, -1]], dtype=torch.complex128),
+             math.sqrt(0.2) * torch.tensor([[0, 1], [0, 0.5]], dtype=torch.complex128)]
+
+
+@pytest.mark.parametrize("phase", [False, True])
+def test_oracle_noisy_hamiltonian_vs_dense(phase):
+    n = 5
+    om, de, ph, u = _params(n, 3, phase)
+    k = orc.noise_from_lindbladians(LINDBLADS)
+    h = orc.OracleHamiltonian(om, de, ph, u, noise=k)
+    dense = orc.dense_noisy_hamiltonian(om, de, ph, u, k)
+    psi = torch.randn(2**n, dtype=torch.complex128)
+    assert torch.allclose(h * psi, dense @ psi, atol=1e-12, rtol=0)
+    assert not torch.allclose(dense, dense.conj().T)  # K makes H non-Hermitian
+
+
+def test_noise_term_closed_form():
+    """K = -i/2 L^+L: Hermitian L^+L, so K is anti-Hermitian, and the
+    product's helper agrees with the oracle's."""
+    from emulators_amd.emu_base.jump_lindblad_operators import compute_noise_from_lindbladians
+    k = compute_noise_from_lindbladians(LINDBLADS)
+    assert torch.allclose(k, orc.noise_from_lindbladians(LINDBLADS), atol=0, rtol=0)
+    assert torch.allclose(k, -k.conj().T, atol=1e-15)
+    assert torch.allclose(1j * (k - k.conj().T),
+                          sum(L.conj().T @ L for L in LINDBLADS), atol=1e-14)
+
+
+def test_oracle_noise_leaves_real_energy():
+    """test_hamiltonian.py:134-172: a diagonal K = diag(0, -i/2) changes only
+    the imaginary part of <psi|H|psi>."""
+    n = 3
+    torch.manual_seed(0)
+    om, de = torch.randn(n, dtype=torch.float64), torch.randn(n, dtype=torch.float64)
+    u = torch.randn(n, n, dtype=torch.float64)
+    u = u + u.T
+    psi = torch.zeros(8, dtype=torch.complex128)
+    psi[0b111] = psi[0b100] = 1 / math.sqrt(2)  # (|rrr> + |rgg>)/sqrt 2
+    h0 = orc.OracleHamiltonian(om, de, torch.zeros(n), u)
+    h = orc.OracleHamiltonian(om, de, torch.zeros(n), u,
+                              noise=torch.tensor([[0.0, 0.0], [0.0, -0.5j]], dtype=torch.complex128))
+    e0 = h0.expect(psi)
+    expected = 0.5 * (-2 * de[0] - de[1] - de[2] + u[0, 1] + u[0, 2] + u[1, 2])
+    assert torch.allclose(e0, expected)
+    assert h.expect(psi) == e0
+
+
+def test_oracle_single_qubit_helpers_vs_dense():
+    n = 4
+    torch.manual_seed(5)
+    psi = torch.randn(2**n, dtype=torch.complex128)
+    ops = torch.randn(3, 2, 2, dtype=torch.complex128)
+    t = orc.expect_batch(psi, ops, n)
+    for q in range(n):
+        for i in range(3):
+            full = orc._single_gate(q, n, ops[i])
+            assert abs(t[q, i] - torch.vdot(psi, full @ psi)) < 1e-12
+        assert torch.allclose(orc.apply_1q(psi, q, ops[0]), orc._single_gate(q, n, ops[0]) @ psi,
+                              atol=1e-13)
+
+
+@pytest.mark.usefixtures("restore_rng_state")
+def test_oracle_monte_carlo_decay_closed_form():
+    """One atom, no drive, relaxation L = sqrt(g)|g><r| from |r>: the norm is
+    exp(-g t), so the jump happens where exp(-g t) equals the threshold, lands
+    in |g> and no further jump follows."""
+    g = 2.0
+    L = [math.sqrt(g) * torch.tensor([[0, 1], [0, 0]], dtype=torch.complex128)]
+    random.seed(11)
+    mc = orc.OracleMonteCarlo()
+    thr = mc.jump_threshold
+    t_jump = -math.log(thr) / g
+    psi = torch.tensor([0, 1], dtype=torch.complex128)
+    dt, t = 0.05, 0.0
+    zero = torch.zeros(1, dtype=torch.float64)
+    while t < t_jump + 3 * dt:
+        before = len(mc.jumps)
+        psi = mc.step(dt, zero, zero, zero, torch.zeros(1, 1), psi, 1e-12, L)
+        t += dt
+        if len(mc.jumps) > before:
+            assert t - dt <= t_jump <= t + 1e-12
+    assert mc.jumps == [0]
+    assert abs(abs(psi[0]) - 1) < 1e-12 and abs(psi[1]) < 1e-12
+
+
+def test_lindblad_solver_is_not_built():
+    """Lindblad noise without stochastic noise selects the density-matrix
+    solver (sv_backend_impl.py:84-93), which this package does not build."""
+    from types import SimpleNamespace
+
+    from emulators_amd.emu_sv.sv_backend_impl import SVBackendImpl
+    from emulators_amd.emu_sv.sv_config import SVConfig
+    data = SimpleNamespace(lindblad_ops=[torch.eye(2, dtype=torch.complex128)],
+                           has_stochastic_noise=False)
+    with pytest.raises(NotImplementedError):
+        SVBackendImpl(SVConfig(observables=[], log_level=40), data)
