@@ -67,6 +67,8 @@ SIGNATURES = {
     "emu_sv_arnoldi_orthogonalize": (_i, [_i, _i, _i, ctypes.POINTER(_vp), _dp, _vp, _vp, _vp,
                                           _vp]),
     "emu_sv_apply_1q": (_i, [_i, _i, _vp, _i, _dp, _vp]),
+    "emu_sv_apply_pair": (_i, [_i, _i, _vp, _vp, _i, _i, _dp, _i, _vp]),
+    "emu_sv_antihermitian": (_i, [_i, _i, _vp, _vp, _d, _d, _vp]),
     "emu_sv_convert": (_i, [_i, _i, _vp, _i, _vp, _vp]),
     "emu_sv_correlation": (_i, [_i, _i, _vp, _vp, _vp, _vp]),
     "emu_sv_parameter_traces": (_i, [_i, _i, _i, _vp, _vp, ctypes.c_uint64, _i, _vp, _vp, _vp]),
@@ -365,6 +367,38 @@ def apply_1q(n: int, psi: torch.Tensor, qubit: int, op) -> torch.Tensor:
                               a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                               _stream(psi.device)), "apply_1q")
     return psi
+
+
+def apply_pair(n: int, src: torch.Tensor, dst: torch.Tensor, bit_a: int, bit_b: int, op,
+               accumulate: bool = True) -> torch.Tensor:
+    """dst (+)= S on the bit pair (bit_a, bit_b), S 4x4 complex over the local
+    index 2*x_a + x_b."""
+    _require_device_vec(src, "apply_pair")
+    _require_device_vec(dst, "apply_pair")
+    if src.dtype != dst.dtype or src.numel() != dst.numel() or src.numel() != (1 << n):
+        raise ValueError("apply_pair: src and dst must be 2**n elements of one dtype")
+    m = np.ascontiguousarray(np.asarray(op, dtype=np.complex128))
+    if m.shape != (4, 4):
+        raise ValueError("apply_pair: the operator must be 4x4")
+    a = np.ascontiguousarray(m.reshape(16)).view(np.float64)
+    check(LIB.emu_sv_apply_pair(n, dtype_code(src.dtype), _ptr(src), _ptr(dst), bit_a, bit_b,
+                                a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                int(accumulate), _stream(src.device)), "apply_pair")
+    return dst
+
+
+def antihermitian(m: torch.Tensor, out: torch.Tensor, s: complex = 1.0) -> torch.Tensor:
+    """out = s (M - M^H) for a square 2^k x 2^k device matrix (out may be M)."""
+    _require_device_vec(m, "antihermitian")
+    _require_device_vec(out, "antihermitian")
+    side = m.shape[0]
+    k = side.bit_length() - 1
+    if m.dim() != 2 or m.shape[1] != side or (1 << k) != side or out.shape != m.shape or \
+            out.dtype != m.dtype:
+        raise ValueError("antihermitian: M and out must be square 2^k x 2^k of one dtype")
+    check(LIB.emu_sv_antihermitian(k, dtype_code(m.dtype), _ptr(m), _ptr(out), float(s.real),
+                                   float(s.imag), _stream(m.device)), "antihermitian")
+    return out
 
 
 def scale(n: int, v: torch.Tensor, s: float) -> None:

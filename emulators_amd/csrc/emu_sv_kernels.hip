@@ -878,6 +878,104 @@ __global__ __launch_bounds__(kThreads) void apply_1q_kernel(uint64_t npairs, voi
   }
 }
 
+// A 4x4 operator on the bit pair (a, b): thread g owns the four amplitudes
+// whose bits a and b take every value, local index 2*bit_a + bit_b.
+struct Op2q {
+  double s[32];  // row-major 4x4, (re, im)
+};
+__device__ __forceinline__ uint64_t insert_zero(uint64_t x, int bit) {
+  const uint64_t lo = x & ((1ull << bit) - 1);
+  return ((x >> bit) << (bit + 1)) | lo;
+}
+template <typename T>
+__global__ __launch_bounds__(kThreads) void pair_op_kernel(uint64_t ngroups, const void *inv,
+                                                           void *outv, int a, int b,
+                                                           const Op2q op, int accumulate) {
+  using T2 = typename Vec2<T>::type;
+  const T2 *in = reinterpret_cast<const T2 *>(inv);
+  T2 *out = reinterpret_cast<T2 *>(outv);
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  for (uint64_t g = (uint64_t)blockIdx.x * kThreads + threadIdx.x; g < ngroups; g += stride) {
+    const uint64_t x = insert_zero(insert_zero(g, lo), hi);
+    uint64_t idx[4];
+    idx[0] = x;
+    idx[1] = x | (1ull << b);
+    idx[2] = x | (1ull << a);
+    idx[3] = x | (1ull << a) | (1ull << b);
+    double vr[4], vi[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const T2 v = in[idx[k]];
+      vr[k] = v.x;
+      vi[k] = v.y;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double yr = 0.0, yi = 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double sr = op.s[2 * (4 * r + c)], si = op.s[2 * (4 * r + c) + 1];
+        yr += sr * vr[c] - si * vi[c];
+        yi += sr * vi[c] + si * vr[c];
+      }
+      if (accumulate) {
+        const T2 o = out[idx[r]];
+        yr += o.x;
+        yi += o.y;
+      }
+      T2 w;
+      w.x = static_cast<T>(yr);
+      w.y = static_cast<T>(yi);
+      out[idx[r]] = w;
+    }
+  }
+}
+
+// out = s (M - M^H) for a 2^k x 2^k row-major matrix M.  Block (bx, by) with
+// by <= bx owns the tile pair (I, J) = (by, bx) and (J, I): both tiles are
+// staged in LDS before either is written, so out may alias M.
+constexpr int kAhTile = 32;
+template <typename T>
+__global__ __launch_bounds__(kThreads) void antiherm_kernel(int side_log, const void *mv, void *outv,
+                                                            double s_re, double s_im) {
+  using T2 = typename Vec2<T>::type;
+  __shared__ T2 ta[kAhTile][kAhTile + 1];
+  __shared__ T2 tb[kAhTile][kAhTile + 1];
+  if (blockIdx.y > blockIdx.x) return;
+  const uint64_t side = 1ull << side_log;
+  const int ts = side < (uint64_t)kAhTile ? (int)side : kAhTile;
+  const uint64_t I = (uint64_t)blockIdx.y * ts, J = (uint64_t)blockIdx.x * ts;
+  const T2 *m = reinterpret_cast<const T2 *>(mv);
+  T2 *out = reinterpret_cast<T2 *>(outv);
+  const int tx = threadIdx.x % kAhTile, ty = threadIdx.x / kAhTile;  // 32 x 8
+  for (int r = ty; r < ts; r += kThreads / kAhTile) {
+    if (tx < ts) {
+      ta[r][tx] = m[(I + r) * side + J + tx];  // tile (I, J)
+      tb[r][tx] = m[(J + r) * side + I + tx];  // tile (J, I)
+    }
+  }
+  __syncthreads();
+  for (int r = ty; r < ts; r += kThreads / kAhTile) {
+    if (tx < ts) {
+      // out[I+r][J+tx] = s (M[I+r][J+tx] - conj(M[J+tx][I+r]))
+      const T2 p = ta[r][tx], q = tb[tx][r];
+      double dr = (double)p.x - (double)q.x, di = (double)p.y + (double)q.y;
+      T2 w;
+      w.x = static_cast<T>(s_re * dr - s_im * di);
+      w.y = static_cast<T>(s_re * di + s_im * dr);
+      out[(I + r) * side + J + tx] = w;
+      // out[J+r][I+tx] = s (M[J+r][I+tx] - conj(M[I+tx][J+r]))
+      const T2 p2 = tb[r][tx], q2 = ta[tx][r];
+      dr = (double)p2.x - (double)q2.x;
+      di = (double)p2.y + (double)q2.y;
+      w.x = static_cast<T>(s_re * dr - s_im * di);
+      w.y = static_cast<T>(s_re * di + s_im * dr);
+      out[(J + r) * side + I + tx] = w;
+    }
+  }
+}
+
 template <typename S, typename D2>
 __global__ __launch_bounds__(kThreads) void convert_kernel(uint64_t D,
                                                            const typename Vec2<S>::type *src,
@@ -2057,6 +2155,44 @@ int emu_sv_apply_1q(int n, int dtype, void *psi, int bit, const double *a, hipSt
     hipLaunchKernelGGL(apply_1q_kernel<float>, dim3(nb), dim3(kThreads), 0, stream, npairs, psi,
                        bit, op);
   return hip_check(hipGetLastError(), "apply_1q launch");
+}
+
+int emu_sv_apply_pair(int n, int dtype, const void *in, void *out, int bit_a, int bit_b,
+                      const double *op, int accumulate, hipStream_t stream) {
+  if (int rc = check_n(n)) return rc;
+  if (int rc = check_dtype(dtype)) return rc;
+  if (!in || !out || !op) return fail(EMU_SV_EINVAL, "apply_pair: null pointer");
+  if (n < 2 || bit_a < 0 || bit_a >= n || bit_b < 0 || bit_b >= n || bit_a == bit_b)
+    return fail(EMU_SV_EINVAL, "apply_pair: bits must be distinct and in [0, n)");
+  Op2q o;
+  for (int k = 0; k < 32; ++k) o.s[k] = op[k];
+  const uint64_t ngroups = 1ull << (n - 2);
+  const int nb = stream_blocks(ngroups);
+  if (dtype == EMU_SV_C128)
+    hipLaunchKernelGGL(pair_op_kernel<double>, dim3(nb), dim3(kThreads), 0, stream, ngroups, in,
+                       out, bit_a, bit_b, o, accumulate ? 1 : 0);
+  else
+    hipLaunchKernelGGL(pair_op_kernel<float>, dim3(nb), dim3(kThreads), 0, stream, ngroups, in,
+                       out, bit_a, bit_b, o, accumulate ? 1 : 0);
+  return hip_check(hipGetLastError(), "apply_pair launch");
+}
+
+int emu_sv_antihermitian(int side_log, int dtype, const void *m, void *out, double s_re,
+                         double s_im, hipStream_t stream) {
+  if (side_log < 0 || side_log > EMU_SV_MAX_QUBITS / 2)
+    return fail(EMU_SV_EINVAL, "antihermitian: side_log must be in [0, 16]");
+  if (int rc = check_dtype(dtype)) return rc;
+  if (!m || !out) return fail(EMU_SV_EINVAL, "antihermitian: null pointer");
+  const uint64_t side = 1ull << side_log;
+  const uint64_t ts = side < (uint64_t)kAhTile ? side : (uint64_t)kAhTile;
+  const unsigned nt = (unsigned)(side / ts);
+  if (dtype == EMU_SV_C128)
+    hipLaunchKernelGGL(antiherm_kernel<double>, dim3(nt, nt), dim3(kThreads), 0, stream, side_log,
+                       m, out, s_re, s_im);
+  else
+    hipLaunchKernelGGL(antiherm_kernel<float>, dim3(nt, nt), dim3(kThreads), 0, stream, side_log,
+                       m, out, s_re, s_im);
+  return hip_check(hipGetLastError(), "antihermitian launch");
 }
 
 int emu_sv_scale(int n, int dtype, void *v, double s, hipStream_t stream) {

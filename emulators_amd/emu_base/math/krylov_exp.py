@@ -398,18 +398,29 @@ def fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
     return _fused_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim, rows)
 
 
-def _fused_arnoldi(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
+def _fused_arnoldi(op, v: torch.Tensor, exp_tolerance: float,
                    norm_tolerance: float, max_krylov_dim: int) -> KrylovExpResult:
     """Arnoldi branch of krylov_exp_impl (krylov_exp.py:44-107 with
-    is_hermitian=False) on the device kernels.  Basis vectors are stored
-    unnormalised with a host-side scale; v itself is basis vector 0."""
-    ham = op.hamiltonian
+    is_hermitian=False) on the device kernels.  ``op`` is a
+    ``ScaledHamiltonian`` or an operator with ``device_matvec(src, scale,
+    dst)`` (dst = op(scale * src) on flat device vectors, e.g. the scaled
+    Lindbladian on a density matrix).  Basis vectors are stored unnormalised
+    with a host-side scale; v itself is basis vector 0."""
+    shape = v.shape
+    v = v.contiguous().view(-1)
     n = native.nqubits_of(v)
-    if n != ham.nqubits:
-        raise ValueError(f"vector has {n} qubits, the Hamiltonian {ham.nqubits}")
-    v = v.contiguous()
-    params = ham.device_params(v.device)
-    sched = ham._schedule(v)
+    if isinstance(op, ScaledHamiltonian):
+        ham = op.hamiltonian
+        if n != ham.nqubits:
+            raise ValueError(f"vector has {n} qubits, the Hamiltonian {ham.nqubits}")
+        params = ham.device_params(v.device)
+        sched = ham._schedule(v)
+
+        def matvec(src, scale, dst):
+            native.hpsi(n, params, op.sigma, src, scale, dst, complex_coef=ham.complex,
+                        noise=ham.noisy, **sched.kwargs())
+    else:
+        matvec = op.device_matvec
     initial_norm = math.sqrt(float(native.dot(n, v)[0].item()))
     basis = [v]
     scales = [1.0 / initial_norm]
@@ -417,13 +428,12 @@ def _fused_arnoldi(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
 
     def combine(coeffs) -> torch.Tensor:
         c = np.asarray(coeffs, dtype=np.complex128) * np.asarray(scales[: len(coeffs)]) * initial_norm
-        return native.lincomb(n, basis[: len(coeffs)], c, torch.empty_like(v))
+        return native.lincomb(n, basis[: len(coeffs)], c, torch.empty_like(v)).view(shape)
 
     expd = None
     for j in range(max_krylov_dim):
         w = torch.empty_like(v)
-        native.hpsi(n, params, op.sigma, basis[j], scales[j], w, complex_coef=ham.complex,
-                    noise=ham.noisy, **sched.kwargs())
+        matvec(basis[j], scales[j], w)
         KrylovStats.matvecs += 1
         rec = native.arnoldi_orthogonalize(n, basis, scales, w).cpu().numpy()
         nrm = math.sqrt(max(rec[0, 0], 0.0))  # ||op(v_j)||, before orthogonalisation
@@ -494,6 +504,12 @@ def krylov_exp_impl(op: Callable[[torch.Tensor], torch.Tensor], v: torch.Tensor,
         run = fused_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim)
         return KrylovExpResult(run.combine(), run.converged, run.happy_breakdown,
                                run.iteration_count)
+    if hasattr(op, "device_matvec"):  # e.g. the scaled Lindbladian
+        if not v.is_cuda:
+            raise ValueError("the fused Krylov engine needs the state on the GPU")
+        if is_hermitian:
+            raise ValueError("this operator runs on the Arnoldi engine: use is_hermitian=False")
+        return _fused_arnoldi(op, v, exp_tolerance, norm_tolerance, max_krylov_dim)
     return _generic_krylov(op, v, is_hermitian, exp_tolerance, norm_tolerance, max_krylov_dim)
 
 

@@ -1,7 +1,9 @@
 """State-vector backend: drop-in for the reference's emu_sv on MI355X."""
 
 from .dense_state import DenseState
+from .density_matrix_state import DensityMatrix
 from .hamiltonian import RydbergHamiltonian
+from .lindblad_operator import RydbergLindbladian
 from .observables import (
     BitStrings,
     CorrelationMatrix,
@@ -22,6 +24,7 @@ __all__ = [
     "BitStrings",
     "CorrelationMatrix",
     "DenseState",
+    "DensityMatrix",
     "Energy",
     "EnergySecondMoment",
     "EnergyVariance",
@@ -30,6 +33,7 @@ __all__ = [
     "Occupation",
     "Results",
     "RydbergHamiltonian",
+    "RydbergLindbladian",
     "SVBackend",
     "SVConfig",
     "Solver",

@@ -65,20 +65,26 @@ class Observable:
         result._store(observable=self, time=t, value=value)
 
 
+def _is_density_matrix(state) -> bool:
+    return getattr(state, "data", state).dim() == 2
+
+
 class Occupation(Observable):
     _base_tag = "occupation"
 
     def apply(self, *, config, state, hamiltonian):
-        return impl.qubit_occupation_sv_impl(self, config=config, state=state,
-                                             hamiltonian=hamiltonian)
+        f = (impl.qubit_occupation_sv_den_mat_impl if _is_density_matrix(state)
+             else impl.qubit_occupation_sv_impl)
+        return f(self, config=config, state=state, hamiltonian=hamiltonian)
 
 
 class CorrelationMatrix(Observable):
     _base_tag = "correlation_matrix"
 
     def apply(self, *, config, state, hamiltonian):
-        return impl.correlation_matrix_sv_impl(self, config=config, state=state,
-                                               hamiltonian=hamiltonian)
+        f = (impl.correlation_matrix_sv_den_mat_impl if _is_density_matrix(state)
+             else impl.correlation_matrix_sv_impl)
+        return f(self, config=config, state=state, hamiltonian=hamiltonian)
 
 
 class Energy(Observable):
@@ -92,16 +98,18 @@ class EnergyVariance(Observable):
     _base_tag = "energy_variance"
 
     def apply(self, *, config, state, hamiltonian):
-        return impl.energy_variance_sv_impl(self, config=config, state=state,
-                                            hamiltonian=hamiltonian)
+        f = (impl.energy_variance_sv_den_mat_impl if _is_density_matrix(state)
+             else impl.energy_variance_sv_impl)
+        return f(self, config=config, state=state, hamiltonian=hamiltonian)
 
 
 class EnergySecondMoment(Observable):
     _base_tag = "energy_second_moment"
 
     def apply(self, *, config, state, hamiltonian):
-        return impl.energy_second_moment_sv_impl(self, config=config, state=state,
-                                                 hamiltonian=hamiltonian)
+        f = (impl.energy_second_moment_den_mat_impl if _is_density_matrix(state)
+             else impl.energy_second_moment_sv_impl)
+        return f(self, config=config, state=state, hamiltonian=hamiltonian)
 
 
 class Fidelity(Observable):

@@ -27,29 +27,31 @@ from emulators_amd.emu_sv.observables import Results, Statistics
 from emulators_amd.emu_sv.state_vector import StateVector
 from emulators_amd.emu_sv.sv_config import SVConfig
 from emulators_amd.emu_sv.sv_config import Solver
-from emulators_amd.emu_sv.time_evolution import EvolveMonteCarlo, EvolveStateVector
+from emulators_amd.emu_sv.time_evolution import (EvolveDensityMatrix, EvolveMonteCarlo,
+                                                 EvolveStateVector)
 from emulators_amd.emu_sv.dense_state import DenseState
 
 _TIME_CONVERSION_COEFF = 0.001  # Omega and delta in rad/us, dt in ns
 
 
+def select_stepper(config: SVConfig, data):
+    """The stepper choice of sv_backend_impl.py:81-95: Lindblad noise runs as
+    Monte-Carlo trajectories when the noise is stochastic (or asked for),
+    else as a density matrix under the Lindbladian.  Returns
+    (stepper, evolves_a_density_matrix)."""
+    if data.lindblad_ops:
+        solver = getattr(config, "solver", Solver.DEFAULT)
+        if (solver == Solver.DEFAULT and data.has_stochastic_noise) or \
+                solver == Solver.MONTECARLO:
+            return EvolveMonteCarlo(config.max_krylov_dim), False
+        return EvolveDensityMatrix(config.max_krylov_dim), True
+    return EvolveStateVector(config.max_krylov_dim), False
+
+
 class SVBackendImpl:
     def __init__(self, config: SVConfig, data: SequenceData):
-        # stepper choice of sv_backend_impl.py:81-95: Lindblad noise runs as
-        # Monte-Carlo trajectories when the noise is stochastic (or asked for);
-        # the density-matrix solver is not part of this package
         self.pulser_lindblads = list(data.lindblad_ops)
-        if self.pulser_lindblads:
-            solver = getattr(config, "solver", Solver.DEFAULT)
-            if (solver == Solver.DEFAULT and data.has_stochastic_noise) or \
-                    solver == Solver.MONTECARLO:
-                self.stepper = EvolveMonteCarlo(config.max_krylov_dim)
-            else:
-                raise NotImplementedError(
-                    "the Lindblad density-matrix solver is not part of the GPU state-vector path; "
-                    "use solver=Solver.MONTECARLO")
-        else:
-            self.stepper = EvolveStateVector(config.max_krylov_dim)
+        self.stepper, self.density_matrix = select_stepper(config, data)
         self._config = config
         self._data = data
         self.target_times = data.target_times
@@ -86,19 +88,36 @@ class SVBackendImpl:
 
     # -- state ---------------------------------------------------------------
     def _init_state(self) -> None:
+        # sv_backend_impl.py:142-160: a state vector is promoted to |psi><psi|
+        # for the density-matrix solver
         cfg_state = self._config.initial_state
         if cfg_state is not None:
-            if not isinstance(cfg_state, DenseState) or cfg_state.data.dim() != 1:
-                raise ValueError("initial_state must be a state vector")
+            if not isinstance(cfg_state, DenseState):
+                raise ValueError("initial_state must be a dense state")
             data = cfg_state.data.detach().clone().to("cuda", torch.complex128)
+            if data.dim() == 2 and not self.density_matrix:
+                raise ValueError("a density-matrix initial state needs the Lindblad solver")
+            if self.density_matrix and data.dim() == 1:
+                data = torch.outer(data, data.conj())
         else:
-            data = torch.zeros(2**self.nqubits, dtype=torch.complex128, device="cuda")
-            data[0] = 1.0
+            side = 2**self.nqubits
+            if self.density_matrix:
+                data = torch.zeros(side, side, dtype=torch.complex128, device="cuda")
+                data[0, 0] = 1.0
+            else:
+                data = torch.zeros(side, dtype=torch.complex128, device="cuda")
+                data[0] = 1.0
+        if self.density_matrix and self._work_dtype != torch.complex128:
+            raise ValueError("the Lindblad solver runs in complex128")
         self._work = data.to(self._work_dtype).contiguous()
 
     @property
-    def state(self) -> StateVector:
-        """The current state as a complex128 StateVector (promoted if needed)."""
+    def state(self):
+        """The current state as a complex128 StateVector, or a DensityMatrix
+        for the Lindblad solver."""
+        if self.density_matrix:
+            from emulators_amd.emu_sv.density_matrix_state import DensityMatrix
+            return DensityMatrix(self._work, gpu=True)
         data = self._work
         if data.dtype != torch.complex128:
             data = native.convert(native.nqubits_of(data), data,
@@ -193,4 +212,4 @@ def run_sequence_data(sequence_data: SequenceData, config: SVConfig) -> Results:
     return SVBackendImpl(config, sequence_data)._run()
 
 
-__all__ = ["SVBackendImpl", "run_sequence_data"]
+__all__ = ["SVBackendImpl", "run_sequence_data", "select_stepper"]

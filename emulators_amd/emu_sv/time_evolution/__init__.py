@@ -1,6 +1,6 @@
 from .interface import BaseStepper
-from .implementations import EvolveMonteCarlo, EvolveStateVector
+from .implementations import EvolveDensityMatrix, EvolveMonteCarlo, EvolveStateVector
 from .torch import DifferentiableStateVectorEvolution
 
-__all__ = ["BaseStepper", "DifferentiableStateVectorEvolution", "EvolveMonteCarlo",
-           "EvolveStateVector"]
+__all__ = ["BaseStepper", "DifferentiableStateVectorEvolution", "EvolveDensityMatrix",
+           "EvolveMonteCarlo", "EvolveStateVector"]

@@ -1,5 +1,6 @@
-"""State-vector steppers (reference emu_sv/time_evolution/implementations.py):
-the noiseless one (:16-60) and the Monte-Carlo quantum-jump one (:123-250)."""
+"""Steppers (reference emu_sv/time_evolution/implementations.py): the
+noiseless state vector (:16-60), the Lindblad density matrix (:63-120) and
+the Monte-Carlo quantum-jump state vector (:123-250)."""
 
 from __future__ import annotations
 
@@ -15,6 +16,7 @@ from emulators_amd.emu_base.math.krylov_exp import (DEFAULT_MAX_KRYLOV_DIM, Scal
                                                      krylov_exp)
 from emulators_amd.emu_sv.algebra import apply, expect_batch
 from emulators_amd.emu_sv.hamiltonian import RydbergHamiltonian
+from emulators_amd.emu_sv.lindblad_operator import RydbergLindbladian, ScaledLindbladian
 from emulators_amd.emu_sv.time_evolution.interface import BaseStepper
 from emulators_amd.emu_sv.time_evolution.torch import DifferentiableStateVectorEvolution
 
@@ -44,6 +46,33 @@ class EvolveStateVector(BaseStepper):
         return DifferentiableStateVectorEvolution.apply(
             dt, omegas, deltas, phis, full_interaction_matrix, state, krylov_tolerance,
             self.max_krylov_dim)
+
+
+class EvolveDensityMatrix(BaseStepper):
+    """rho <- exp(-i dt L) rho for the Lindbladian L (implementations.py:63-120):
+    the device Arnoldi engine on the 4^N-element density matrix."""
+
+    def __init__(self, max_krylov_dim: int = DEFAULT_MAX_KRYLOV_DIM) -> None:
+        super().__init__(max_krylov_dim)
+
+    @staticmethod
+    def get_hamiltonian(omegas, deltas, phis, pulser_lindblads, interaction_matrix,
+                        device) -> RydbergLindbladian:
+        return RydbergLindbladian(omegas=omegas, deltas=deltas, phis=phis,
+                                  pulser_lindblads=pulser_lindblads,
+                                  interaction_matrix=interaction_matrix, device=device)
+
+    def apply(self, dt: float, omegas: torch.Tensor, deltas: torch.Tensor, phis: torch.Tensor,
+              full_interaction_matrix: torch.Tensor, state: torch.Tensor,
+              krylov_tolerance: float, pulser_lindblads: list):
+        ham = self.get_hamiltonian(omegas=omegas, deltas=deltas, phis=phis,
+                                   pulser_lindblads=pulser_lindblads,
+                                   interaction_matrix=full_interaction_matrix,
+                                   device=state.device)
+        out = krylov_exp(ScaledLindbladian(ham, -1j * dt), state,
+                         norm_tolerance=krylov_tolerance, exp_tolerance=krylov_tolerance,
+                         is_hermitian=False, max_krylov_dim=self.max_krylov_dim)
+        return out, ham
 
 
 JUMP_TIME_TOLERANCE_FRACTION = 0.1

@@ -224,6 +224,22 @@ int emu_sv_arnoldi_orthogonalize(int n, int dtype, int count, const void *const 
 int emu_sv_apply_1q(int n, int dtype, void *psi, int bit, const double *a,
                     hipStream_t stream);
 
+/* out (+)= S acting on the bit pair (bit_a, bit_b): S a 4x4 complex matrix
+ * (32 doubles row-major (re, im), host) over the local index
+ * 2*x_{bit_a} + x_{bit_b}; accumulate != 0 adds to out.  in may alias out.
+ * On a density matrix stored as a 2n-bit vector (row bits high), the pair
+ * (row bit, column bit) of one qubit carries K (x) I and L (x) conj(L), the
+ * jump terms L rho L^+ of the Lindbladian (emu_sv/lindblad_operator.py:94-135,
+ * 203-213). */
+int emu_sv_apply_pair(int n, int dtype, const void *in, void *out, int bit_a,
+                      int bit_b, const double *op, int accumulate, hipStream_t stream);
+
+/* out = s (M - M^H) for a 2^side_log x 2^side_log row-major complex matrix M
+ * (s complex); out may alias M.  The anti-Hermitian part taken twice in
+ * RydbergLindbladian.__matmul__ (emu_sv/lindblad_operator.py:196-216). */
+int emu_sv_antihermitian(int side_log, int dtype, const void *m, void *out,
+                         double s_re, double s_im, hipStream_t stream);
+
 /* Copy with element-type conversion (c128 <-> c64 or same type). */
 int emu_sv_convert(int n, int src_dtype, const void *src, int dst_dtype,
                    void *dst, hipStream_t stream);

@@ -724,3 +724,86 @@ class OracleMonteCarlo:
             state = self.jump(state, lindblads, aggregated, n)
             gap = state.norm().item() ** 2 - self.jump_threshold
         return state
+
+
+# ---------------------------------------------------------------------------
+# Lindblad density-matrix path  (emu_sv/lindblad_operator.py,
+# emu_sv/time_evolution/implementations.py:63-120)
+# ---------------------------------------------------------------------------
+class OracleLindbladian:
+    """Restates RydbergLindbladian (lindblad_operator.py:11-223) on CPU
+    torch: local 2x2 terms applied through (2^q, 2, rest) views of rho (rows)
+    and (2^(q+N), 2, rest) views (columns), the interaction diagonal on the
+    rows, then the anti-Hermitian parts of :196-216."""
+
+    def __init__(self, omegas, deltas, phis, pulser_lindblads, interaction_matrix):
+        self.nqubits = len(omegas)
+        self.omegas = torch.as_tensor(omegas).detach().to(DTYPE) / 2.0
+        self.deltas = torch.as_tensor(deltas).detach().to(DTYPE)
+        self.phis = torch.as_tensor(phis).detach().to(DTYPE)
+        self.lindblads = [torch.as_tensor(L, dtype=DTYPE) for L in pulser_lindblads]
+        self.u = torch.as_tensor(interaction_matrix).detach()
+        self.complex = bool(self.phis.any())
+        n = self.nqubits
+        diag = torch.zeros(2**n, dtype=DTYPE)  # :66-83, interaction only
+        for i in range(n):
+            diag = diag.view(2**i, 2, -1)
+            i_fixed = diag[:, 1, :]
+            for j in range(i + 1, n):
+                i_fixed = i_fixed.view(2**i, 2 ** (j - i - 1), 2, -1)
+                i_fixed[:, :, 1, :] += self.u[i, j].item()
+        self.diag = diag.view(-1)
+
+    def left(self, rho, op, q):  # :94-113
+        return (op @ rho.reshape(2**q, 2, -1)).reshape(rho.shape)
+
+    def right_dag(self, rho, op, q):  # :115-135, rho @ op^H on qubit q
+        return (op.conj() @ rho.reshape(2 ** (q + self.nqubits), 2, -1)).reshape(rho.shape)
+
+    def local_h(self, q, k):  # :158-180
+        sx = torch.tensor([[0.0, 1.0], [1.0, 0.0]], dtype=DTYPE)
+        sy = torch.tensor([[0.0, -1j], [1j, 0.0]], dtype=DTYPE)
+        nop = torch.tensor([[0.0, 0.0], [0.0, 1.0]], dtype=DTYPE)
+        drive = torch.cos(self.phis[q]) * sx + torch.sin(self.phis[q]) * sy if self.complex else sx
+        return self.omegas[q] * drive - self.deltas[q] * nop + k
+
+    def h_eff(self, rho, k=None):  # :137-156
+        k = torch.zeros(2, 2, dtype=DTYPE) if k is None else k
+        out = torch.zeros_like(rho)
+        for q in range(self.nqubits):
+            out += self.left(rho, self.local_h(q, k), q)
+        return out + self.diag.view(-1, 1) * rho
+
+    def __matmul__(self, rho):  # :182-216
+        k = noise_from_lindbladians(self.lindblads)
+        h = self.h_eff(rho, k)
+        h = h - h.conj().T
+        jumps = torch.zeros_like(rho)
+        for q in range(self.nqubits):
+            for L in self.lindblads:
+                jumps += self.right_dag(self.left(rho, L, q), L, q)
+        res = h + 1.0j * jumps
+        return 0.5 * (res - res.conj().T)
+
+    def expect(self, rho):  # :218-223
+        return self.h_eff(rho).trace().real
+
+
+def dense_lindblad_superoperator(omegas, deltas, phis, interaction_matrix, lindblads):
+    """The Liouvillian as a 4^N x 4^N matrix on row-major vec(rho):
+    d rho/dt = -i [H, rho] + sum_k (A rho A^H - 1/2 {A^H A, rho}), the
+    textbook Lindblad form that -i * (L @ rho) must equal."""
+    h = dense_rydberg_hamiltonian(omegas, deltas, phis, interaction_matrix)
+    n = len(omegas)
+    d = 2**n
+    eye = torch.eye(d, dtype=DTYPE)
+    # row-major vec: vec(A X B) = (A kron B^T) vec(X)
+    sup = -1j * (torch.kron(h, eye) - torch.kron(eye, h.T.contiguous()))
+    for L in lindblads:
+        L = torch.as_tensor(L, dtype=DTYPE)
+        for q in range(n):
+            a = _single_gate(q, n, L)
+            ada = a.conj().T @ a
+            sup += torch.kron(a, a.conj()) - 0.5 * (torch.kron(ada, eye)
+                                                   + torch.kron(eye, ada.T.contiguous()))
+    return sup

@@ -202,14 +202,22 @@ def test_oracle_monte_carlo_decay_closed_form():
     assert abs(abs(psi[0]) - 1) < 1e-12 and abs(psi[1]) < 1e-12
 
 
-def test_lindblad_solver_is_not_built():
-    """Lindblad noise without stochastic noise selects the density-matrix
-    solver (sv_backend_impl.py:84-93), which this package does not build."""
+@pytest.mark.parametrize("solver,ops,stochastic,expected", [
+    ("default", [], False, "EvolveStateVector"),
+    ("default", [1], True, "EvolveMonteCarlo"),
+    ("default", [1], False, "EvolveDensityMatrix"),
+    ("montecarlo", [1], False, "EvolveMonteCarlo"),
+    ("lindblad", [1], True, "EvolveDensityMatrix"),
+])
+def test_stepper_selection(solver, ops, stochastic, expected):
+    """The solver table of sv_backend_impl.py:84-95 (the reference's
+    test_sv_backend.py:158-240)."""
     from types import SimpleNamespace
 
-    from emulators_amd.emu_sv.sv_backend_impl import SVBackendImpl
-    from emulators_amd.emu_sv.sv_config import SVConfig
-    data = SimpleNamespace(lindblad_ops=[torch.eye(2, dtype=torch.complex128)],
-                           has_stochastic_noise=False)
-    with pytest.raises(NotImplementedError):
-        SVBackendImpl(SVConfig(observables=[], log_level=40), data)
+    from emulators_amd.emu_sv.sv_backend_impl import select_stepper
+    from emulators_amd.emu_sv.sv_config import Solver, SVConfig
+    data = SimpleNamespace(lindblad_ops=[torch.eye(2, dtype=torch.complex128)] * len(ops),
+                           has_stochastic_noise=stochastic)
+    stepper, dm = select_stepper(SVConfig(observables=[], log_level=40, solver=Solver(solver)), data)
+    assert type(stepper).__name__ == expected
+    assert dm == (expected == "EvolveDensityMatrix")
