@@ -69,6 +69,9 @@ SIGNATURES = {
     "emu_sv_apply_1q": (_i, [_i, _i, _vp, _i, _dp, _vp]),
     "emu_sv_apply_pair": (_i, [_i, _i, _vp, _vp, _i, _i, _dp, _i, _vp]),
     "emu_sv_antihermitian": (_i, [_i, _i, _vp, _vp, _d, _d, _vp]),
+    "emu_sv_sample_workspace_bytes": (ctypes.c_size_t, [_i]),
+    "emu_sv_sample": (_i, [_i, _i, _vp, ctypes.c_uint64, _i, _vp, _i, _vp, _vp, ctypes.c_size_t,
+                           _vp]),
     "emu_sv_convert": (_i, [_i, _i, _vp, _i, _vp, _vp]),
     "emu_sv_correlation": (_i, [_i, _i, _vp, _vp, _vp, _vp]),
     "emu_sv_parameter_traces": (_i, [_i, _i, _i, _vp, _vp, ctypes.c_uint64, _i, _vp, _vp, _vp]),
@@ -398,6 +401,31 @@ def antihermitian(m: torch.Tensor, out: torch.Tensor, s: complex = 1.0) -> torch
         raise ValueError("antihermitian: M and out must be square 2^k x 2^k of one dtype")
     check(LIB.emu_sv_antihermitian(k, dtype_code(m.dtype), _ptr(m), _ptr(out), float(s.real),
                                    float(s.imag), _stream(m.device)), "antihermitian")
+    return out
+
+
+def sample(n: int, src: torch.Tensor, uniforms: torch.Tensor, density_matrix: bool = False
+           ) -> torch.Tensor:
+    """Basis indices drawn by inverse CDF from |psi|^2 (state vector) or
+    |diag(rho)| (density matrix), one per uniform draw in [0, 1) (device
+    float64).  Returns a device int64 tensor."""
+    _require_device_vec(src, "sample")
+    side = 1 << n
+    if density_matrix:
+        if src.numel() != side * side:
+            raise ValueError("sample: the density matrix must be 2**n x 2**n")
+        stride, mode = side + 1, 1
+    else:
+        if src.numel() != side:
+            raise ValueError("sample: the state must have 2**n elements")
+        stride, mode = 1, 0
+    u = uniforms.to(device=src.device, dtype=torch.float64).contiguous()
+    out = torch.empty(u.numel(), dtype=torch.int64, device=src.device)
+    nbytes = LIB.emu_sv_sample_workspace_bytes(n)
+    ws = torch.empty(max(nbytes, 8), dtype=torch.uint8, device=src.device)
+    check(LIB.emu_sv_sample(n, dtype_code(src.dtype), _ptr(src), ctypes.c_uint64(stride), mode,
+                            _ptr(u), u.numel(), _ptr(out), _ptr(ws), ctypes.c_size_t(ws.numel()),
+                            _stream(src.device)), "sample")
     return out
 
 

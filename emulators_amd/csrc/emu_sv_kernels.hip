@@ -976,6 +976,146 @@ __global__ __launch_bounds__(kThreads) void antiherm_kernel(int side_log, const 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Inverse-CDF sampling of basis indices.  p(x) = |src[x]|^2 (mode 0) or
+// |src[x * stride]| (mode 1, the diagonal of a density matrix).  The vector
+// is cut into chunks of kSampleChunk elements; a wavefront owns a chunk with
+// lane l summing its kSampleChunk/64 consecutive elements, and the same
+// decomposition is used when a sample descends into a chunk, so the chunk
+// sums and the in-chunk search add in one order.
+// ---------------------------------------------------------------------------
+constexpr int kSampleChunkLog = 12;
+
+template <typename T>
+__device__ __forceinline__ double sample_prob(const void *src, uint64_t stride, int mode,
+                                              uint64_t x) {
+  using T2 = typename Vec2<T>::type;
+  const T2 v = reinterpret_cast<const T2 *>(src)[x * stride];
+  const double re = v.x, im = v.y;
+  return mode == 0 ? re * re + im * im : sqrt(re * re + im * im);
+}
+
+// Lane l's element range of chunk c (chunk of min(D, 2^12) elements).
+__device__ __forceinline__ void sample_lane_range(uint64_t D, uint64_t c, int lane, uint64_t &x0,
+                                                  uint64_t &x1) {
+  const uint64_t chunk = D < (1ull << kSampleChunkLog) ? D : (1ull << kSampleChunkLog);
+  const uint64_t per = chunk >= 64 ? chunk / 64 : 1;
+  const uint64_t base = c * chunk;
+  x0 = base + (uint64_t)lane * per;
+  x1 = x0 + per;
+  if (x0 > base + chunk) x0 = base + chunk;
+  if (x1 > base + chunk) x1 = base + chunk;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void sample_chunk_kernel(uint64_t D, uint64_t nchunks,
+                                                                const void *src, uint64_t stride,
+                                                                int mode, double *sums) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t c = (uint64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (c >= nchunks) return;
+  uint64_t x0, x1;
+  sample_lane_range(D, c, lane, x0, x1);
+  double s = 0.0;
+  for (uint64_t x = x0; x < x1; ++x) s += sample_prob<T>(src, stride, mode, x);
+  s = wave_sum(s);
+  if (lane == 0) sums[c] = s;
+}
+
+// Inclusive prefix sums of the chunk sums in place (one workgroup; thread t
+// owns a run of consecutive chunks).
+__global__ __launch_bounds__(kThreads) void sample_scan_kernel(uint64_t nchunks, double *sums) {
+  __shared__ double part[kThreads];
+  const uint64_t per = (nchunks + kThreads - 1) / kThreads;
+  const uint64_t c0 = threadIdx.x * per;
+  const uint64_t c1 = c0 + per < nchunks ? c0 + per : nchunks;
+  double s = 0.0;
+  for (uint64_t c = c0; c < c1; ++c) s += sums[c];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double acc = 0.0;
+    for (int t = 0; t < kThreads; ++t) {
+      const double v = part[t];
+      part[t] = acc;  // exclusive offset of thread t's run
+      acc += v;
+    }
+  }
+  __syncthreads();
+  double acc = part[threadIdx.x];
+  for (uint64_t c = c0; c < c1; ++c) {
+    acc += sums[c];
+    sums[c] = acc;
+  }
+}
+
+// One wavefront per draw u in [0, 1): the first index x whose cumulative
+// probability exceeds u * total (searchsorted(cdf, u, right=True)).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void sample_draw_kernel(uint64_t D, uint64_t nchunks,
+                                                               const void *src, uint64_t stride,
+                                                               int mode, const double *ends,
+                                                               const double *uniforms, int count,
+                                                               int64_t *out) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (k >= count) return;
+  const double total = ends[nchunks - 1];
+  const double target = uniforms[k] * total;
+  // first chunk whose inclusive end exceeds the target
+  uint64_t lo = 0, hi = nchunks;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) / 2;
+    if (ends[mid] > target) hi = mid;
+    else lo = mid + 1;
+  }
+  if (lo >= nchunks) lo = nchunks - 1;
+  const uint64_t c = lo;
+  const double r = target - (c > 0 ? ends[c - 1] : 0.0);
+  uint64_t x0, x1;
+  sample_lane_range(D, c, lane, x0, x1);
+  double s = 0.0;
+  for (uint64_t x = x0; x < x1; ++x) s += sample_prob<T>(src, stride, mode, x);
+  // inclusive prefix over lanes
+  double pre = s;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double o = __shfl_up(pre, off, 64);
+    if (lane >= off) pre += o;
+  }
+  const unsigned long long hit = __ballot(pre > r);
+  int64_t found = -1;
+  if (hit) {
+    const int l = __ffsll((long long)hit) - 1;
+    if (lane == l) {
+      double run = pre - s;
+      for (uint64_t x = x0; x < x1; ++x) {
+        run += sample_prob<T>(src, stride, mode, x);
+        if (run > r) {
+          found = (int64_t)x;
+          break;
+        }
+      }
+      if (found < 0) found = (int64_t)(x1 > x0 ? x1 - 1 : x0);
+    }
+    found = __shfl((long long)found, l, 64);
+  } else {
+    // rounding left r at or above the chunk's own sum: its last element
+    // with non-zero probability
+    uint64_t cx0, cx1;
+    sample_lane_range(D, c, 0, cx0, cx1);
+    const uint64_t chunk = D < (1ull << kSampleChunkLog) ? D : (1ull << kSampleChunkLog);
+    int64_t last = (int64_t)(c * chunk + chunk - 1);
+    for (int64_t x = last; x >= (int64_t)(c * chunk); --x)
+      if (sample_prob<T>(src, stride, mode, (uint64_t)x) > 0.0) {
+        last = x;
+        break;
+      }
+    found = last;
+  }
+  if (lane == 0) out[k] = found < (int64_t)D ? found : (int64_t)D - 1;
+}
+
 template <typename S, typename D2>
 __global__ __launch_bounds__(kThreads) void convert_kernel(uint64_t D,
                                                            const typename Vec2<S>::type *src,
@@ -2193,6 +2333,50 @@ int emu_sv_antihermitian(int side_log, int dtype, const void *m, void *out, doub
     hipLaunchKernelGGL(antiherm_kernel<float>, dim3(nt, nt), dim3(kThreads), 0, stream, side_log,
                        m, out, s_re, s_im);
   return hip_check(hipGetLastError(), "antihermitian launch");
+}
+
+size_t emu_sv_sample_workspace_bytes(int n) {
+  if (n < 1 || n > EMU_SV_MAX_QUBITS) return 0;
+  const uint64_t D = 1ull << n;
+  const uint64_t chunk = D < (1ull << kSampleChunkLog) ? D : (1ull << kSampleChunkLog);
+  return (size_t)(D / chunk) * sizeof(double);
+}
+
+int emu_sv_sample(int n, int dtype, const void *src, uint64_t stride, int mode,
+                  const double *uniforms, int count, int64_t *indices_out, void *workspace,
+                  size_t workspace_bytes, hipStream_t stream) {
+  if (int rc = check_n(n)) return rc;
+  if (int rc = check_dtype(dtype)) return rc;
+  if (!src || !uniforms || !indices_out || !workspace)
+    return fail(EMU_SV_EINVAL, "sample: null pointer");
+  if (mode != 0 && mode != 1) return fail(EMU_SV_EINVAL, "sample: mode must be 0 or 1");
+  if (stride < 1) return fail(EMU_SV_EINVAL, "sample: stride must be >= 1");
+  if (count < 0) return fail(EMU_SV_EINVAL, "sample: count must be >= 0");
+  if (workspace_bytes < emu_sv_sample_workspace_bytes(n))
+    return fail(EMU_SV_EINVAL, "sample: workspace too small");
+  if (count == 0) return EMU_SV_OK;
+  const uint64_t D = 1ull << n;
+  const uint64_t chunk = D < (1ull << kSampleChunkLog) ? D : (1ull << kSampleChunkLog);
+  const uint64_t nchunks = D / chunk;
+  double *sums = reinterpret_cast<double *>(workspace);
+  const unsigned g1 = (unsigned)((nchunks + kWaves - 1) / kWaves);
+  const unsigned g3 = (unsigned)((count + kWaves - 1) / kWaves);
+  if (dtype == EMU_SV_C128)
+    hipLaunchKernelGGL(sample_chunk_kernel<double>, dim3(g1), dim3(kThreads), 0, stream, D,
+                       nchunks, src, stride, mode, sums);
+  else
+    hipLaunchKernelGGL(sample_chunk_kernel<float>, dim3(g1), dim3(kThreads), 0, stream, D,
+                       nchunks, src, stride, mode, sums);
+  if (int rc = hip_check(hipGetLastError(), "sample chunk launch")) return rc;
+  hipLaunchKernelGGL(sample_scan_kernel, dim3(1), dim3(kThreads), 0, stream, nchunks, sums);
+  if (int rc = hip_check(hipGetLastError(), "sample scan launch")) return rc;
+  if (dtype == EMU_SV_C128)
+    hipLaunchKernelGGL(sample_draw_kernel<double>, dim3(g3), dim3(kThreads), 0, stream, D,
+                       nchunks, src, stride, mode, sums, uniforms, count, indices_out);
+  else
+    hipLaunchKernelGGL(sample_draw_kernel<float>, dim3(g3), dim3(kThreads), 0, stream, D,
+                       nchunks, src, stride, mode, sums, uniforms, count, indices_out);
+  return hip_check(hipGetLastError(), "sample draw launch");
 }
 
 int emu_sv_scale(int n, int dtype, void *v, double s, hipStream_t stream) {

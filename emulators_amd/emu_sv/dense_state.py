@@ -35,12 +35,23 @@ class DenseState:
 
     def sample(self, num_shots: int = 1000, one_state=None, p_false_pos: float = 0.0,
                p_false_neg: float = 0.0) -> Counter:
-        """Bitstring counts drawn from |psi|^2 (dense_state.py:40-95)."""
-        if self.data.dim() == 2:
-            probabilities = torch.abs(self.data.diagonal())
+        """Bitstring counts drawn from |psi|^2 or |diag(rho)| (dense_state.py:40-95).
+
+        On the GPU the indices come from the native inverse-CDF sampler
+        (emu_sv_sample) fed with torch.rand draws; on the CPU from
+        ``sample_indices``.  Either way the distribution is the reference's;
+        seeded draws differ from torch.multinomial's."""
+        if self.data.is_cuda:
+            from emulators_amd import _native as native
+            u = torch.rand(num_shots, dtype=torch.float64, device=self.data.device)
+            outcomes = native.sample(self.n_qudits, self.data.contiguous(), u,
+                                     density_matrix=self.data.dim() == 2).cpu()
         else:
-            probabilities = torch.abs(self.data) ** 2
-        outcomes = sample_indices(probabilities, num_shots)
+            if self.data.dim() == 2:
+                probabilities = torch.abs(self.data.diagonal())
+            else:
+                probabilities = torch.abs(self.data) ** 2
+            outcomes = sample_indices(probabilities, num_shots)
         counts = Counter(index_to_bitstring(self.n_qudits, int(o)) for o in outcomes.tolist())
         if p_false_neg > 0 or p_false_pos > 0:
             counts = apply_measurement_errors(counts, p_false_pos=p_false_pos,

@@ -240,6 +240,21 @@ int emu_sv_apply_pair(int n, int dtype, const void *in, void *out, int bit_a,
 int emu_sv_antihermitian(int side_log, int dtype, const void *m, void *out,
                          double s_re, double s_im, hipStream_t stream);
 
+/* Bytes of device workspace emu_sv_sample needs for 2^n categories. */
+size_t emu_sv_sample_workspace_bytes(int n);
+
+/* Inverse-CDF sampling of basis indices: for each of count draws u_k in
+ * [0, 1) (uniforms, device doubles), indices_out[k] (device int64) = the first
+ * x whose cumulative probability exceeds u_k * sum(p), clamped to 2^n - 1
+ * (torch.searchsorted(cdf, u, right=True) of emu_sv/utils.py:17-49).
+ * p(x) = |src[x]|^2 (mode 0, a state vector) or |src[x * stride]| (mode 1,
+ * the diagonal of a density matrix with stride 2^n + 1).  FP64 sums; the
+ * draws themselves come from the caller.  Replaces the multinomial /
+ * inverse-CDF sampling of DenseState.sample (emu_sv/dense_state.py:40-95). */
+int emu_sv_sample(int n, int dtype, const void *src, uint64_t stride, int mode,
+                  const double *uniforms, int count, int64_t *indices_out,
+                  void *workspace, size_t workspace_bytes, hipStream_t stream);
+
 /* Copy with element-type conversion (c128 <-> c64 or same type). */
 int emu_sv_convert(int n, int src_dtype, const void *src, int dst_dtype,
                    void *dst, hipStream_t stream);

@@ -807,3 +807,15 @@ def dense_lindblad_superoperator(omegas, deltas, phis, interaction_matrix, lindb
             sup += torch.kron(a, a.conj()) - 0.5 * (torch.kron(ada, eye)
                                                    + torch.kron(eye, ada.T.contiguous()))
     return sup
+
+
+# ---------------------------------------------------------------------------
+# Sampling  (emu_sv/utils.py:17-49, emu_sv/dense_state.py:40-95)
+# ---------------------------------------------------------------------------
+def inverse_cdf_indices(probabilities: torch.Tensor, uniforms: torch.Tensor) -> torch.Tensor:
+    """searchsorted(cdf, u * total, right=True) clamped to the last index: the
+    inverse-CDF branch of sample_indices (utils.py:36-49), with the draws
+    given."""
+    cdf = torch.cumsum(probabilities.to(torch.float64), dim=0)
+    idx = torch.searchsorted(cdf, uniforms.to(torch.float64) * cdf[-1], right=True)
+    return idx.clamp_(max=probabilities.numel() - 1)
