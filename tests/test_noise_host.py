@@ -221,3 +221,55 @@ def test_stepper_selection(solver, ops, stochastic, expected):
     stepper, dm = select_stepper(SVConfig(observables=[], log_level=40, solver=Solver(solver)), data)
     assert type(stepper).__name__ == expected
     assert dm == (expected == "EvolveDensityMatrix")
+
+
+def _noise_model(**kw):
+    from types import SimpleNamespace
+    base = dict(noise_types=(), relaxation_rate=0.0, dephasing_rate=0.0,
+                hyperfine_dephasing_rate=0.0, depolarizing_rate=0.0, eff_noise_rates=(),
+                eff_noise_opers=())
+    base.update(kw)
+    return SimpleNamespace(**base)
+
+
+def test_lindblad_operators_eff_noise_basis():
+    """test_lindblad_operators.py:12-40: pulser's |0><1| in the ising basis
+    becomes sqrt(rate) |g><r| in the state-vector basis; in XY it is kept."""
+    from emulators_amd.emu_base.jump_lindblad_operators import get_lindblad_operators
+    for interaction, b0, b1 in (("ising", [0.0, 1.0], [1.0, 0.0]), ("XY", [1.0, 0.0], [0.0, 1.0])):
+        basis0 = torch.tensor(b0, dtype=torch.complex128).reshape(2, 1)
+        basis1 = torch.tensor(b1, dtype=torch.complex128).reshape(2, 1)
+        nm = _noise_model(noise_types=("eff_noise",), eff_noise_rates=[0.5],
+                          eff_noise_opers=[basis0 @ basis1.T])
+        ops = get_lindblad_operators(noise_type="eff_noise", noise_model=nm,
+                                     interact_type=interaction)
+        assert torch.allclose(ops[0], math.sqrt(0.5) * torch.tensor(
+            [[0.0, 1.0], [0.0, 0.0]], dtype=torch.complex128))
+
+
+def test_lindblad_operators_channels():
+    """Relaxation / dephasing / depolarizing closed forms
+    (jump_lindblad_operators.py:18-54); non-Lindbladian noise types are
+    skipped, an unknown one raises."""
+    from emulators_amd.emu_base.jump_lindblad_operators import (get_all_lindblad_noise_operators,
+                                                                get_lindblad_operators)
+    nm = _noise_model(noise_types=("relaxation", "dephasing", "depolarizing", "SPAM", "doppler"),
+                      relaxation_rate=0.4, dephasing_rate=0.6, depolarizing_rate=0.8)
+    ops = get_all_lindblad_noise_operators(nm)
+    assert len(ops) == 5
+    assert torch.allclose(ops[0], torch.tensor([[0, math.sqrt(0.4)], [0, 0]], dtype=torch.complex128))
+    assert torch.allclose(ops[1], math.sqrt(0.3) * torch.diag(torch.tensor([1.0, -1.0])).to(torch.complex128))
+    c = math.sqrt(0.2)
+    assert torch.allclose(ops[2], c * torch.tensor([[0, 1], [1, 0]], dtype=torch.complex128))
+    assert torch.allclose(ops[3], c * torch.tensor([[0, -1j], [1j, 0]], dtype=torch.complex128))
+    # sum of L^+L of the depolarizing channel = 3/4 rate * I
+    s = sum(L.conj().T @ L for L in ops[2:])
+    assert torch.allclose(s, 0.6 * torch.eye(2, dtype=torch.complex128))
+    with pytest.raises(ValueError, match="Only 2 by 2"):
+        get_lindblad_operators(noise_type="eff_noise", noise_model=_noise_model(
+            noise_types=("eff_noise",), eff_noise_rates=[0.5], eff_noise_opers=[torch.eye(3)]))
+    with pytest.raises(ValueError):
+        get_lindblad_operators(noise_type="nope", noise_model=_noise_model(noise_types=("nope",)))
+    with pytest.raises(NotImplementedError):
+        get_lindblad_operators(noise_type="dephasing", noise_model=_noise_model(
+            noise_types=("dephasing",), hyperfine_dephasing_rate=1.0))
