@@ -48,7 +48,7 @@ def get_lindblad_operators(*, noise_type: str, noise_model, interact_type: str =
         z[0, 0], z[1, 1] = c, -c
         return [x, y, z]
     if noise_type == "eff_noise":
-        ops = [torch.tensor(op, dtype=dtype) for op in noise_model.eff_noise_opers]
+        ops = [torch.as_tensor(op).detach().clone().to(dtype) for op in noise_model.eff_noise_opers]
         if not all(tuple(op.shape) == (dim, dim) for op in ops):
             raise ValueError(f"Only {dim} by {dim} effective noise operator matrices are supported")
         out = [math.sqrt(rate) * op for rate, op in zip(noise_model.eff_noise_rates, ops)]
