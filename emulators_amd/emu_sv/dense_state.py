@@ -100,6 +100,7 @@ class DenseState:
                               amplitudes: Mapping[str, complex]) -> DenseStateType:
         """State from a {bitstring: amplitude} mapping, normalised."""
         n = len(next(iter(amplitudes)))
-        state, _ = cls._from_state_amplitudes(eigenstates=eigenstates, n_qudits=n,
-                                              amplitudes=amplitudes)
+        state, amps = cls._from_state_amplitudes(eigenstates=eigenstates, n_qudits=n,
+                                                 amplitudes=amplitudes)
+        state._amplitudes = dict(amps)  # kept for the config's serialised form
         return state

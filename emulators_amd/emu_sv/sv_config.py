@@ -11,13 +11,20 @@ state-vector path reads (``observables``, ``default_evaluation_times``,
 One option is new: ``precision`` = "complex128" (default, the reference's
 arithmetic) or "complex64" (state and Krylov vectors stored in complex64;
 Lanczos coefficients, reductions and the small exponential stay FP64).
+
+``to_abstract_repr`` / ``from_abstract_repr`` serialise a config as JSON
+(sv_config.py and pulser's EmulationConfig): the keyword arguments, the
+observables (type, tag suffix, evaluation times and their own options) and
+an initial state as eigenstates plus {bitstring: amplitude}.  Complex
+amplitudes are written as [re, im] pairs; pulser's own schema is not
+available here, so the wire format matches this package's reader only.
 """
 
 from __future__ import annotations
 
+import json
 import logging
 import pathlib
-import warnings
 from enum import Enum
 from typing import Any, Sequence
 
@@ -51,9 +58,10 @@ class SVConfig:
                  initial_state=None, interaction_matrix=None, with_modulation: bool = False,
                  noise_model=None, **kwargs: Any):
         unexpected = set(kwargs) - self._expected
-        if unexpected:
-            warnings.warn(f"SVConfig: unexpected keyword arguments {sorted(unexpected)}",
-                          stacklevel=2)
+        if unexpected:  # the message of pulser's EmulationConfig
+            raise ValueError(
+                f"'{type(self).__name__}' received unexpected keyword arguments: {unexpected}; "
+                f"only the following keyword arguments are expected: {self._expected}")
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {PRECISIONS}, got {precision!r}")
         if dt <= 0:
@@ -94,3 +102,73 @@ class SVConfig:
 
     def is_time_in_evaluation_times(self, t: float, times: Sequence[float], tol: float = 1e-10) -> bool:
         return any(abs(t - float(x)) <= tol for x in times)
+
+
+    # -- serialised form ------------------------------------------------------
+    _SCALARS = ("dt", "max_krylov_dim", "krylov_tolerance", "gpu", "interaction_cutoff",
+                "log_level", "precision", "with_modulation")
+
+    def to_abstract_repr(self) -> str:
+        """The config as a JSON string (see the module docstring)."""
+        d: dict = {k: getattr(self, k) for k in self._SCALARS}
+        d["log_file"] = None if self.log_file is None else str(self.log_file)
+        d["solver"] = Solver(self.solver).value
+        d["default_evaluation_times"] = (self.default_evaluation_times
+                                         if isinstance(self.default_evaluation_times, str)
+                                         else list(self.default_evaluation_times))
+        d["observables"] = [_observable_repr(o) for o in self.observables]
+        d["initial_state"] = None if self.initial_state is None else _state_repr(self.initial_state)
+        return json.dumps(d)
+
+    @classmethod
+    def from_abstract_repr(cls, text: str) -> "SVConfig":
+        d = json.loads(text)
+        kw = {k: d[k] for k in cls._SCALARS if k in d}
+        kw["log_file"] = None if d.get("log_file") is None else pathlib.Path(d["log_file"])
+        kw["solver"] = Solver(d.get("solver", "default"))
+        kw["default_evaluation_times"] = d.get("default_evaluation_times", (1.0,))
+        kw["observables"] = [_observable_from_repr(o) for o in d.get("observables", [])]
+        if d.get("initial_state") is not None:
+            kw["initial_state"] = _state_from_repr(d["initial_state"])
+        return cls(**kw)
+
+
+def _state_repr(state) -> dict:
+    amps = getattr(state, "_amplitudes", None)
+    if amps is None:  # from the nonzero amplitudes of the tensor
+        data = state.data.detach().cpu()
+        if data.dim() != 1:
+            raise ValueError("only state vectors can be serialised")
+        n = int(data.numel()).bit_length() - 1
+        amps = {format(i, f"0{n}b").replace("1", "r").replace("0", "g"): complex(v)
+                for i, v in enumerate(data.tolist()) if v != 0}
+    return {"eigenstates": list(state.eigenstates),
+            "amplitudes": {k: [complex(v).real, complex(v).imag] for k, v in amps.items()}}
+
+
+def _state_from_repr(d: dict):
+    from emulators_amd.emu_sv.state_vector import StateVector
+    amps = {k: complex(v[0], v[1]) for k, v in d["amplitudes"].items()}
+    return StateVector.from_state_amplitudes(eigenstates=tuple(d["eigenstates"]), amplitudes=amps)
+
+
+def _observable_repr(obs) -> dict:
+    d = {"type": type(obs).__name__, "tag_suffix": obs.tag_suffix,
+         "evaluation_times": None if obs.evaluation_times is None
+         else [float(t) for t in obs.evaluation_times]}
+    if hasattr(obs, "num_shots"):
+        d["num_shots"] = obs.num_shots
+    if type(obs).__name__ == "Fidelity":
+        d["state"] = _state_repr(obs.state)
+    return d
+
+
+def _observable_from_repr(d: dict):
+    from emulators_amd.emu_sv import observables as obs_mod
+    cls = getattr(obs_mod, d["type"])
+    kw = {"evaluation_times": d.get("evaluation_times"), "tag_suffix": d.get("tag_suffix")}
+    if "num_shots" in d:
+        kw["num_shots"] = d["num_shots"]
+    if d["type"] == "Fidelity":
+        return cls(_state_from_repr(d["state"]), **kw)
+    return cls(**kw)

@@ -260,7 +260,7 @@ def test_sv_config_defaults_and_validation():
         SVConfig(precision="bf16")
     with pytest.raises(ValueError):
         SVConfig(observables=[Occupation(), Occupation()], log_level=40)
-    with pytest.warns(UserWarning):
+    with pytest.raises(ValueError, match="unexpected keyword arguments"):
         SVConfig(not_an_option=1, log_level=40)
 
 
