@@ -379,6 +379,120 @@ def parity_legs() -> dict:
     return out
 
 
+def noise_legs() -> dict:
+    """The noisy paths on the GPU.
+
+    monte_carlo: 10-atom chain (the chain10 pulse, 100 steps) with relaxation
+    (1 /us) and dephasing (0.5 /us); four seeded quantum-jump trajectories
+    (Python ``random`` seeds 0..3).  lindblad: the same pulse and noise on 6
+    atoms as a density matrix (2^12 elements).  The final states are kept
+    (``_states``) for the CPU-baseline leg, which replays the same workloads
+    on the oracle and reports the agreement."""
+    import random
+
+    import torch
+
+    from emulators_amd import workloads
+    from emulators_amd.emu_sv import Occupation, StateResult, SVBackend, SVConfig
+
+    lind = workloads.relaxation_dephasing(1.0, 0.5)
+    cfg = SVConfig(observables=[StateResult(evaluation_times=[1.0]),
+                                Occupation(evaluation_times=[1.0])], log_level=40)
+    out = {"_states": {}}
+    seq = workloads.noisy_chain(10, lind, stochastic=True)
+    times = []
+    for seed in range(4):
+        random.seed(seed)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = SVBackend._run_from_sequence_data(seq, cfg)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        if seed == 0:
+            out["_states"]["monte_carlo"] = res.state[-1].data.cpu()
+    out["monte_carlo"] = {
+        "workload": "10-atom chain, 5 um, Omega 4pi, delta -10 -> 10 rad/us over 1000 ns "
+                    "(100 steps), relaxation 1/us + dephasing 0.5/us, complex128",
+        "trajectories": len(times), "seconds_per_trajectory": sum(times[1:]) / (len(times) - 1),
+        "first_trajectory_s": times[0]}
+    n = 6
+    seq = workloads.noisy_chain(n, lind, stochastic=False)
+    SVBackend._run_from_sequence_data(workloads.noisy_chain(n, lind, False, duration=50), cfg)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = SVBackend._run_from_sequence_data(seq, cfg)
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
+    out["_states"]["lindblad"] = (res.state[-1].data.cpu(), torch.as_tensor(res.occupation[-1]))
+    out["lindblad"] = {
+        "workload": f"{n}-atom chain density matrix (2^{2 * n} elements), the same pulse and noise, "
+                    "100 steps, complex128", "seconds": t_gpu}
+    log(f"noise: {out['monte_carlo']['seconds_per_trajectory']:.2f} s per Monte-Carlo trajectory "
+        f"(N=10), {t_gpu:.2f} s for the N=6 density matrix")
+    return out
+
+
+def cpu_noise_baseline(noise: dict) -> dict:
+    """The noisy workloads of ``noise_legs`` on the oracle (host cores):
+    trajectory seed 0 (same draws, so the same jumps) and the density matrix,
+    timed, with the agreement of the GPU results."""
+    import random
+
+    import torch
+
+    from emulators_amd import workloads
+    from oracle import emu_sv_oracle as orc
+
+    lind = workloads.relaxation_dephasing(1.0, 0.5)
+
+    def loop(seq, state, step_fn):
+        tt = seq.target_times
+        for k in range(seq.omega.shape[0]):
+            dt = (tt[k + 1] - tt[k]) * 1e-3
+            state = step_fn(dt, seq.omega[k].real, seq.delta[k].real, seq.phi[k].real,
+                            seq.interaction_matrix(tt[k])[0], state)
+        return state
+
+    out = {}
+    seq = workloads.noisy_chain(10, lind, stochastic=True)
+    random.seed(0)
+    mc = orc.OracleMonteCarlo()
+    psi0 = torch.zeros(2**10, dtype=torch.complex128)
+    psi0[0] = 1.0
+    t0 = time.perf_counter()
+    psi = loop(seq, psi0, lambda dt, om, de, ph, u, st: mc.step(dt, om, de, ph, u, st, 1e-10, lind))
+    t_mc = time.perf_counter() - t0
+    psi = psi / psi.norm()
+    a = noise["_states"]["monte_carlo"]
+    a = a / a.norm()
+    out["monte_carlo"] = {"seconds_per_trajectory": t_mc, "jumps": len(mc.jumps),
+                          "fidelity_deficit_gpu_vs_oracle": 1.0 - float(torch.abs(torch.vdot(a, psi)) ** 2)}
+    n = 6
+    seq = workloads.noisy_chain(n, lind, stochastic=False)
+    rho0 = torch.zeros(2**n, 2**n, dtype=torch.complex128)
+    rho0[0, 0] = 1.0
+
+    def dm_step(dt, om, de, ph, u, rho):
+        lo = orc.OracleLindbladian(om, de, ph, lind, u)
+        return orc.krylov_exp(lambda x: -1j * dt * (lo @ x), rho, 1e-10, 1e-10, is_hermitian=False)
+
+    t0 = time.perf_counter()
+    rho = loop(seq, rho0, dm_step)
+    t_dm = time.perf_counter() - t0
+    rho = rho / rho.trace()
+    p = rho.diagonal().real
+    occ = torch.tensor([float(sum(p[x] for x in range(2**n) if (x >> (n - 1 - q)) & 1))
+                        for q in range(n)], dtype=torch.float64)
+    rho_gpu, occ_gpu = noise["_states"]["lindblad"]
+    out["lindblad"] = {"seconds": t_dm,
+                       "max_abs_rho_error_gpu_vs_oracle": float((rho_gpu / rho_gpu.trace() - rho).abs().max()),
+                       "max_occupation_error_gpu_vs_oracle": float((occ_gpu - occ).abs().max())}
+    log(f"CPU baseline, noisy paths: Monte-Carlo trajectory {t_mc:.1f} s ({len(mc.jumps)} jumps, "
+        f"1-F vs GPU {out['monte_carlo']['fidelity_deficit_gpu_vs_oracle']:.1e}), density matrix "
+        f"{t_dm:.1f} s (occupation error {out['lindblad']['max_occupation_error_gpu_vs_oracle']:.1e})")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -392,6 +506,8 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-noise", action="store_true")
+    ap.add_argument("--only-noise", action="store_true", help="time only the noisy paths")
     ap.add_argument("--only-batch", action="store_true", help="time only the batching workload")
     ap.add_argument("--only-adjoint", action="store_true", help="time only the gradient workload")
     ap.add_argument("--only-matvec", action="store_true",
@@ -419,6 +535,13 @@ def main() -> int:
 
     if args.only_batch:
         print(json.dumps(batch_throughput()), flush=True)
+        return 0
+    if args.only_noise:
+        noise = noise_legs()
+        if not args.no_cpu_baseline:
+            noise["cpu_baseline"] = cpu_noise_baseline(noise)
+        noise.pop("_states")
+        print(json.dumps(noise), flush=True)
         return 0
     if args.only_adjoint:
         print(json.dumps(adjoint_cost()), flush=True)
@@ -471,12 +594,17 @@ def main() -> int:
     adjoint = None if args.no_adjoint else adjoint_cost()
     batch = None if args.no_batch else batch_throughput()
     parity = None if args.no_parity else parity_legs()
+    noise = None if args.no_noise else noise_legs()
     mean_iters = mv128 / steps
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle) ...")
         cpu = cpu_baseline(seq, mean_iters, mid, args.cpu_iters)
         log(f"CPU baseline {cpu['value']:.4f} steps/s on {cpu['cores']} threads")
+        if noise is not None:
+            cpu["noise"] = cpu_noise_baseline(noise)
+    if noise is not None:
+        noise.pop("_states")
 
     out = {
         "metric": METRIC,
@@ -517,6 +645,7 @@ def main() -> int:
         "adjoint": adjoint,
         "batch": batch,
         "parity": parity,
+        "noise": noise,
     }
     print(json.dumps(out), flush=True)
     return 0

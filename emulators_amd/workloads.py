@@ -178,3 +178,26 @@ def batch_sequence(inst: dict, config=None) -> SequenceData:
     d = inst["duration"]
     return adiabatic_sequence(inst["positions"], np.full(d, inst["omega"]),
                               ramp(inst["delta_start"], inst["delta_end"], d), config, config.dt)
+
+
+def relaxation_dephasing(gamma_r: float, gamma_d: float) -> list:
+    """Jump operators sqrt(g_r)|g><r| and sqrt(g_d/2) Z (relaxation and
+    dephasing channels, (g, r) basis)."""
+    import torch
+    relax = torch.zeros(2, 2, dtype=torch.complex128)
+    relax[0, 1] = math.sqrt(gamma_r)
+    deph = torch.zeros(2, 2, dtype=torch.complex128)
+    deph[0, 0], deph[1, 1] = math.sqrt(gamma_d / 2), -math.sqrt(gamma_d / 2)
+    return [relax, deph]
+
+
+def noisy_chain(n: int, lindblads: list, stochastic: bool, duration: int = 1000,
+                config=None) -> SequenceData:
+    """The chain10 pulse (5 um spacing, Omega 4 pi, delta -10 -> 10 over
+    ``duration`` ns) on n atoms with Lindblad noise: stochastic=True selects
+    Monte-Carlo trajectories, False the density matrix."""
+    import dataclasses
+    config = config or _config()
+    seq = adiabatic_sequence(chain_positions(n, 5.0), np.full(duration, 4 * np.pi),
+                             ramp(-10, 10, duration), config, config.dt)
+    return dataclasses.replace(seq, lindblad_ops=list(lindblads), has_stochastic_noise=stochastic)
