@@ -131,6 +131,46 @@ struct PassArgs {
 // previous matvec (5 doubles).
 constexpr int kOrthoSums = 5;
 
+// The arithmetic of load_input on already loaded v, pa[x], pb[x].
+template <typename T>
+__device__ __forceinline__ typename Vec2<T>::type finish_input(const PassArgs &a, uint64_t x,
+                                                               typename Vec2<T>::type v,
+                                                               typename Vec2<T>::type pa,
+                                                               typename Vec2<T>::type pb,
+                                                               double (&osum)[kOrthoSums]) {
+  using T2 = typename Vec2<T>::type;
+  T2 z;
+  if (!a.ortho) {
+    const T vs = static_cast<T>(a.v_scale);
+    z.x = v.x * vs;
+    z.y = v.y * vs;
+    return z;
+  }
+  const double wr = a.v_scale * static_cast<double>(v.x), wi = a.v_scale * static_cast<double>(v.y);
+  double zr = wr, zi = wi;
+  if (a.pa) {
+    zr -= a.ca_re * pa.x - a.ca_im * pa.y;
+    zi -= a.ca_re * pa.y + a.ca_im * pa.x;
+  }
+  double br = 0.0, bi = 0.0;
+  if (a.pb) {
+    br = a.pb_scale * static_cast<double>(pb.x);
+    bi = a.pb_scale * static_cast<double>(pb.y);
+    zr -= a.cb_re * pb.x - a.cb_im * pb.y;
+    zi -= a.cb_re * pb.y + a.cb_im * pb.x;
+  }
+  z.x = static_cast<T>(zr);
+  z.y = static_cast<T>(zi);
+  reinterpret_cast<T2 *>(a.zout)[x] = z;
+  const double qr = z.x, qi = z.y;
+  osum[0] += qr * qr + qi * qi;
+  osum[1] += qr * br + qi * bi;
+  osum[2] += qr * bi - qi * br;
+  osum[3] += qr * wr + qi * wi;
+  osum[4] += qr * wi - qi * wr;
+  return z;
+}
+
 template <typename T>
 __device__ __forceinline__ typename Vec2<T>::type load_input(const PassArgs &a, uint64_t x,
                                                              double (&osum)[kOrthoSums]) {
@@ -206,9 +246,37 @@ __device__ __forceinline__ void pass_tile(const PassArgs &a, uint64_t g, const d
 
   const double *__restrict__ H = a.ham;
 
-  // Stage the tile.
+  // Stage the tile.  Whole groups of four elements per thread issue all
+  // their global loads before any arithmetic (several loads in flight).
   double osum[kOrthoSums] = {0, 0, 0, 0, 0};
-  for (uint32_t l = tid; l < tsize; l += kThreads) tile[l] = load_input<T>(a, gidx(l), osum);
+  constexpr int SG = 4;
+  if (tsize % (SG * kThreads) == 0) {
+    const T2 *__restrict__ vin = reinterpret_cast<const T2 *>(a.v);
+    const T2 *__restrict__ pav = reinterpret_cast<const T2 *>(a.pa);
+    const T2 *__restrict__ pbv = reinterpret_cast<const T2 *>(a.pb);
+    for (uint32_t l0 = tid; l0 < tsize; l0 += SG * kThreads) {
+      T2 rv[SG], ra[SG], rb[SG];
+      uint64_t xs[SG];
+#pragma unroll
+      for (int u = 0; u < SG; ++u) {
+        xs[u] = gidx(l0 + u * kThreads);
+        rv[u] = vin[xs[u]];
+      }
+      if (a.ortho && pav) {
+#pragma unroll
+        for (int u = 0; u < SG; ++u) ra[u] = pav[xs[u]];
+      }
+      if (a.ortho && pbv) {
+#pragma unroll
+        for (int u = 0; u < SG; ++u) rb[u] = pbv[xs[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < SG; ++u)
+        tile[l0 + u * kThreads] = finish_input<T>(a, xs[u], rv[u], ra[u], rb[u], osum);
+    }
+  } else {
+    for (uint32_t l = tid; l < tsize; l += kThreads) tile[l] = load_input<T>(a, gidx(l), osum);
+  }
   if (a.diag) {
     // Outer-bit constants of this tile.
     if (tid < t) s_ap[tid] = row[1 + tid];
@@ -295,70 +363,104 @@ __device__ __forceinline__ void pass_tile(const PassArgs &a, uint64_t g, const d
   const T ps = static_cast<T>(a.prev_scale);
   double dots[7] = {0, 0, 0, 0, 0, 0, 0};
 
-  if (tid < (1 << tb)) {
-    for (uint32_t r = 0; r < (1u << k); ++r) {
-      const uint32_t l = (uint32_t)tid | (r << tb);
-      const T2 psi = tile[l];
-      T hr = 0, hi = 0;
+  // One output amplitude: flips from LDS, the diagonal, sigma, the partial
+  // sum o (when accumulating) and the dots with p = prev[x].
+  auto emit = [&](uint32_t r, const T2 o, const T2 pv) {
+    const uint32_t l = (uint32_t)tid | (r << tb);
+    const T2 psi = tile[l];
+    T hr = 0, hi = 0;
 #pragma unroll
-      for (int i = 0; i < kMaxTileLog; ++i) {
-        if ((fm_local >> i) & 1u) {
-          const T2 nb = tile[l ^ (1u << i)];
-          if (NOISE) {
-            const bool up = (l >> i) & 1u;
-            const T fr = up ? cre[i] : dre[i], fi = up ? cim[i] : dim[i];
-            hr += fr * nb.x - fi * nb.y;
-            hi += fr * nb.y + fi * nb.x;
-          } else if (CPLX) {
-            // bit set: <1|H|0> = c ; bit clear: <0|H|1> = conj(c)
-            const T ci = ((l >> i) & 1u) ? cim[i] : -cim[i];
-            hr += cre[i] * nb.x - ci * nb.y;
-            hi += cre[i] * nb.y + ci * nb.x;
-          } else {
-            hr += cre[i] * nb.x;
-            hi += cre[i] * nb.y;
-          }
-        }
-      }
-      const uint64_t x = gidx(l);
-      if (a.diag) {
-        double e = dbase + s_B[r];
-#pragma unroll
-        for (int j = 0; j < kMaxTileLog - kThreadBits; ++j)
-          if ((r >> j) & 1u) e += Gj[j];
+    for (int i = 0; i < kMaxTileLog; ++i) {
+      if ((fm_local >> i) & 1u) {
+        const T2 nb = tile[l ^ (1u << i)];
         if (NOISE) {
-          const double pc = (double)__popcll(x);
-          const double er = e + k0r + pc * k1r, ei = k0i + pc * k1i;
-          const double pr = static_cast<double>(psi.x), pi = static_cast<double>(psi.y);
-          hr += static_cast<T>(er * pr - ei * pi);
-          hi += static_cast<T>(er * pi + ei * pr);
+          const bool up = (l >> i) & 1u;
+          const T fr = up ? cre[i] : dre[i], fi = up ? cim[i] : dim[i];
+          hr += fr * nb.x - fi * nb.y;
+          hi += fr * nb.y + fi * nb.x;
+        } else if (CPLX) {
+          // bit set: <1|H|0> = c ; bit clear: <0|H|1> = conj(c)
+          const T ci = ((l >> i) & 1u) ? cim[i] : -cim[i];
+          hr += cre[i] * nb.x - ci * nb.y;
+          hi += cre[i] * nb.y + ci * nb.x;
         } else {
-          hr += static_cast<T>(e * static_cast<double>(psi.x));
-          hi += static_cast<T>(e * static_cast<double>(psi.y));
+          hr += cre[i] * nb.x;
+          hi += cre[i] * nb.y;
         }
       }
-      T2 w;
-      w.x = sr * hr - si * hi;
-      w.y = sr * hi + si * hr;
-      if (a.accumulate) {
-        const T2 o = out[x];
-        w.x += o.x;
-        w.y += o.y;
+    }
+    const uint64_t x = gidx(l);
+    if (a.diag) {
+      double e = dbase + s_B[r];
+#pragma unroll
+      for (int j = 0; j < kMaxTileLog - kThreadBits; ++j)
+        if ((r >> j) & 1u) e += Gj[j];
+      if (NOISE) {
+        const double pc = (double)__popcll(x);
+        const double er = e + k0r + pc * k1r, ei = k0i + pc * k1i;
+        const double pr = static_cast<double>(psi.x), pi = static_cast<double>(psi.y);
+        hr += static_cast<T>(er * pr - ei * pi);
+        hi += static_cast<T>(er * pi + ei * pr);
+      } else {
+        hr += static_cast<T>(e * static_cast<double>(psi.x));
+        hi += static_cast<T>(e * static_cast<double>(psi.y));
       }
-      out[x] = w;
-      if (DOTS) {
-        const double wr = w.x, wi = w.y, vr = psi.x, vi = psi.y;
-        dots[0] += wr * wr + wi * wi;
-        dots[1] += vr * wr + vi * wi;
-        dots[2] += vr * wi - vi * wr;
-        if (prevp) {
-          T2 p = prevp[x];
-          const double pr = static_cast<double>(p.x * ps), pi = static_cast<double>(p.y * ps);
-          dots[3] += pr * wr + pi * wi;
-          dots[4] += pr * wi - pi * wr;
-          dots[5] += vr * pr + vi * pi;
-          dots[6] += vr * pi - vi * pr;
+    }
+    T2 w;
+    w.x = sr * hr - si * hi;
+    w.y = sr * hi + si * hr;
+    if (a.accumulate) {
+      w.x += o.x;
+      w.y += o.y;
+    }
+    out[x] = w;
+    if (DOTS) {
+      const double wr = w.x, wi = w.y, vr = psi.x, vi = psi.y;
+      dots[0] += wr * wr + wi * wi;
+      dots[1] += vr * wr + vi * wi;
+      dots[2] += vr * wi - vi * wr;
+      if (prevp) {
+        const double pr = static_cast<double>(pv.x * ps), pi = static_cast<double>(pv.y * ps);
+        dots[3] += pr * wr + pi * wi;
+        dots[4] += pr * wi - pi * wr;
+        dots[5] += vr * pr + vi * pi;
+        dots[6] += vr * pi - vi * pr;
+      }
+    }
+  };
+
+  if (tid < (1 << tb)) {
+    const uint32_t nr = 1u << k;
+    T2 zero;
+    zero.x = zero.y = 0;
+    if (nr % SG == 0) {
+      // groups of SG amplitudes: the partial-sum and prev loads first
+      for (uint32_t r0 = 0; r0 < nr; r0 += SG) {
+        T2 o[SG], pv[SG];
+        uint64_t xs[SG];
+#pragma unroll
+        for (int u = 0; u < SG; ++u) {
+          xs[u] = gidx((uint32_t)tid | ((r0 + u) << tb));
+          o[u] = zero;
+          pv[u] = zero;
         }
+        if (a.accumulate) {
+#pragma unroll
+          for (int u = 0; u < SG; ++u) o[u] = out[xs[u]];
+        }
+        if (DOTS && prevp) {
+#pragma unroll
+          for (int u = 0; u < SG; ++u) pv[u] = prevp[xs[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < SG; ++u) emit(r0 + u, o[u], pv[u]);
+      }
+    } else {
+      for (uint32_t r = 0; r < nr; ++r) {
+        const uint64_t x = gidx((uint32_t)tid | (r << tb));
+        const T2 o = a.accumulate ? out[x] : zero;
+        const T2 pv = (DOTS && prevp) ? prevp[x] : zero;
+        emit(r, o, pv);
       }
     }
   }
