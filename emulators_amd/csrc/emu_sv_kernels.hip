@@ -1931,10 +1931,14 @@ int batch_layout(int count, const emu_sv_batch_register *regs, int dtype, int tl
 }
 
 int batch_tile_logs(int dtype, int &tl, int &cmin) {
-  if (tl <= 0) tl = dtype == EMU_SV_C64 ? 11 : 10;
-  if (cmin <= 0) cmin = dtype == EMU_SV_C64 ? 3 : 2;
-  if (tl < kThreadBits || tl > 12 || cmin < 1 || cmin >= tl)
-    return fail(EMU_SV_EINVAL, "batch: tile_log must be in [8, 12], min_contig in [1, tile_log)");
+  // defaults measured on the 256-register workload (N 8..20): 2^12 / 4 for
+  // complex64, 2^11 / 2 for complex128 (32 KiB tiles)
+  if (tl <= 0) tl = dtype == EMU_SV_C64 ? 12 : 11;
+  if (cmin <= 0) cmin = dtype == EMU_SV_C64 ? 4 : 2;
+  const int tmax = dtype == EMU_SV_C64 ? 14 : 13;  // 128 KiB of LDS
+  if (tl < kThreadBits || tl > tmax || cmin < 1 || cmin >= tl)
+    return fail(EMU_SV_EINVAL, "batch: tile_log must be in [8, 13] (complex64: 14), "
+                               "min_contig in [1, tile_log)");
   return EMU_SV_OK;
 }
 
@@ -2677,8 +2681,20 @@ int emu_sv_batch_create(int count, int dtype, const emu_sv_batch_register *regs,
   h->ncomb = L.ncomb;
   h->comb = tables + coff;
   h->lds = esz << max_t;
+  int rc = EMU_SV_OK;
+  if (h->lds > 65536) {  // dynamic LDS above 64 KiB is enabled per kernel
+    rc = hip_check(hipFuncSetAttribute(dtype == EMU_SV_C128
+                                           ? reinterpret_cast<const void *>(&batch_pass_kernel<double>)
+                                           : reinterpret_cast<const void *>(&batch_pass_kernel<float>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds),
+                   "batch_create: LDS attribute");
+    if (rc) {
+      delete h;
+      return rc;
+    }
+  }
   std::vector<double> arr_zero((size_t)count * BatchArr::kTotal, 0.0);
-  int rc = hip_check(hipMemcpyAsync(h->regs, h->host_regs.data(), sizeof(BatchReg) * count,
+  rc = hip_check(hipMemcpyAsync(h->regs, h->host_regs.data(), sizeof(BatchReg) * count,
                                     hipMemcpyHostToDevice, stream), "batch_create: upload");
   if (!rc) rc = hip_check(hipMemcpyAsync(arrays, arr_zero.data(), sizeof(double) * arr_zero.size(),
                                          hipMemcpyHostToDevice, stream), "batch_create: upload");

@@ -308,7 +308,8 @@ size_t emu_sv_batch_workspace_bytes(int count, int dtype, const emu_sv_batch_reg
  * the Expokit test of emu_sv_lanczos_exp) step after step
  * (emu_sv/sv_backend_impl.py:186-209), with its whole state machine in device
  * memory, so one batched iteration is a fixed kernel sequence (captured once
- * in a hipGraph).  max_dim <= 62; tile_log 8..12 / min_contig: the H.psi pass
+ * in a hipGraph).  max_dim <= 62; tile_log 8..13 (complex64: 8..14) /
+ * min_contig: the H.psi pass
  * split (0 = default).  The library allocates no device memory: the
  * workspace (256-byte aligned, zero-filled not required) is the caller's.  *handle_out is a
  * host object released by emu_sv_batch_destroy.  No counterpart in the
