@@ -17,6 +17,7 @@ promoted to a complex128 StateVector only when an observable needs it.
 
 from __future__ import annotations
 
+import os
 import time
 
 import torch
@@ -194,10 +195,82 @@ class SVBackendImpl:
                         self.results)
         self.time = time.time()
 
+    # -- device-driven segments ------------------------------------------------
+    # Between two observable evaluations (and while U(t) stays the same), the
+    # steps of a small noiseless register run as one register of the batch
+    # engine: the Lanczos coefficients, the convergence test and the step
+    # transitions stay on the device, with no host round trip per iteration.
+    # The arithmetic is the per-step engine's (same iteration counts); a
+    # segment that needs more than 62 Krylov vectors is redone step by step.
+    SEGMENT_MAX_QUBITS = 14
+    SEGMENT_MAX_STEPS = 512
+
+    def _use_segments(self) -> bool:
+        if os.environ.get("EMU_SV_SEGMENTS", "1") == "0":
+            return False
+        if not isinstance(self.stepper, EvolveStateVector) or self.nqubits > self.SEGMENT_MAX_QUBITS:
+            return False
+        if torch.is_grad_enabled() and any(getattr(t, "requires_grad", False)
+                                           for t in (self.omega, self.delta, self.phi)):
+            return False
+        return True
+
+    def _segment_end(self, k0: int) -> int:
+        u0 = self.interaction_matrix(self.target_times[k0])
+        k1 = k0 + 1
+        while k1 < self.nsteps and k1 - k0 < self.SEGMENT_MAX_STEPS:
+            if self._callbacks_at(self.target_times[k1] / self.target_times[-1]):
+                break
+            if not torch.equal(self.interaction_matrix(self.target_times[k1]), u0):
+                break
+            k1 += 1
+        return k1
+
+    def _run_segment(self, k0: int, k1: int) -> bool:
+        import numpy as np
+
+        from emulators_amd.emu_sv.batch import BatchRegister, evolve_batch
+
+        tt = self.target_times
+        u = self.interaction_matrix(tt[k0])[0]
+        reg = BatchRegister(
+            omegas=self.omega[k0:k1].detach().cpu().numpy(),
+            deltas=self.delta[k0:k1].detach().cpu().numpy(),
+            phis=self.phi[k0:k1].detach().cpu().numpy(),
+            interaction_matrix=np.asarray(u.detach().cpu().numpy()),
+            dts=np.asarray([(tt[k + 1] - tt[k]) * _TIME_CONVERSION_COEFF for k in range(k0, k1)]),
+            psi0=self._work)
+        res = evolve_batch([reg], dtype=self._work_dtype, tolerance=self._config.krylov_tolerance,
+                           max_krylov_dim=min(self._config.max_krylov_dim, 62),
+                           device=self._work.device)
+        if res.failed:
+            return False
+        self._work = res.states[0]
+        self._current_H = self.stepper.get_hamiltonian(
+            omegas=self.omega[k1 - 1], deltas=self.delta[k1 - 1], phis=self.phi[k1 - 1],
+            pulser_lindblads=self.pulser_lindblads, interaction_matrix=u,
+            device=self._work.device)
+        per_step = (time.time() - self.time) / (k1 - k0)
+        for k in range(k0 + 1, k1 + 1):
+            if k == k1:
+                self._apply_observables(k)
+            self.statistics.data.append(per_step)
+            self.statistics(self._config, tt[k] / tt[-1], _StatsView(self._work),
+                            self._current_H, self.results)
+        self.time = time.time()
+        return True
+
     def _run(self) -> Results:
         self._apply_observables(0)
-        for step in range(self.nsteps):
-            self.step(step)
+        segments = self._use_segments()
+        step = 0
+        while step < self.nsteps:
+            end = self._segment_end(step) if segments else step + 1
+            if end - step > 1 and self._run_segment(step, end):
+                step = end
+            else:
+                self.step(step)
+                step += 1
         return self.results
 
 
