@@ -202,7 +202,7 @@ class SVBackendImpl:
     # transitions stay on the device, with no host round trip per iteration.
     # The arithmetic is the per-step engine's (same iteration counts); a
     # segment that needs more than 62 Krylov vectors is redone step by step.
-    SEGMENT_MAX_QUBITS = 14
+    SEGMENT_MAX_QUBITS = 20
     SEGMENT_MAX_STEPS = 512
 
     def _use_segments(self) -> bool:
