@@ -205,13 +205,13 @@ class PulserData:
     def __init__(self, *, sequence, config, dt: float):
         try:
             import pulser  # noqa: F401
-        except ImportError as exc:  # pragma: no cover - pulser absent in this image
+        except ImportError as exc:  # pragma: no cover - pulser not installed
             raise ImportError(
                 "SVBackend.run() on a pulser Sequence needs pulser; use "
                 "SVBackend._run_from_sequence_data(SequenceData, SVConfig) or "
                 "emulators_amd.emu_base.sequence_data_from_samples instead"
             ) from exc
         raise NotImplementedError(
-            "pulser Sequence sampling is not part of this build; build SequenceData with "
+            "pulser Sequence sampling is not part of emulators_amd; build SequenceData with "
             "sequence_data_from_samples"
         )
