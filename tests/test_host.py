@@ -293,7 +293,7 @@ def test_sampling_helpers():
     assert set(sample_indices(p, 50, max_categories=8).tolist()) == {3}
 
 
-def test_backend_refuses_cpu_and_noise():
+def test_backend_refuses_cpu():
     from emulators_amd import workloads
     from emulators_amd.emu_sv import SVBackend, SVConfig
     seq = workloads.chain10()
@@ -382,3 +382,43 @@ def test_batch_engine_rejects_cpu_and_bad_registers():
     bad = (_Reg * 1)()  # null pointers: the workspace query refuses it
     assert native.LIB.emu_sv_batch_workspace_bytes(1, native.EMU_SV_C64, bad, 0, 0) == 0
     assert native.LIB.emu_sv_batch_destroy(None) == 0
+
+
+def test_new_entry_points_validate_arguments_on_the_host():
+    """Argument checks of the noisy-path, density-matrix and sampling entry
+    points fail before any device work (null device pointers never reach a
+    kernel)."""
+    import ctypes
+
+    from emulators_amd import _native as native
+    lib, ok = native.LIB, native.EMU_SV_OK
+    dp = ctypes.POINTER(ctypes.c_double)
+    eight = (ctypes.c_double * 8)(*([0.0] * 8))
+    block = np.zeros(native.EMU_SV_HAM_DOUBLES)
+    assert lib.emu_sv_pack_noise(eight, block.ctypes.data_as(dp)) == ok
+    assert block[1121] == 1.0
+    bad = (ctypes.c_double * 8)(float("nan"), *([0.0] * 7))
+    assert lib.emu_sv_pack_noise(bad, block.ctypes.data_as(dp)) == native.EMU_SV_EINVAL
+    fake = ctypes.c_void_p(4096)  # never dereferenced: every call below fails validation
+    op32 = (ctypes.c_double * 32)()
+    assert lib.emu_sv_apply_pair(4, 0, fake, fake, 1, 1, op32, 1, None) == native.EMU_SV_EINVAL
+    assert lib.emu_sv_apply_pair(4, 0, fake, fake, 0, 4, op32, 1, None) == native.EMU_SV_EINVAL
+    assert lib.emu_sv_apply_1q(4, 0, fake, 4, eight, None) == native.EMU_SV_EINVAL
+    assert lib.emu_sv_antihermitian(17, 0, fake, fake, 1.0, 0.0, None) == native.EMU_SV_EINVAL
+    vecs = (ctypes.c_void_p * 1)(fake)
+    sc = (ctypes.c_double * 1)(1.0)
+    assert lib.emu_sv_arnoldi_orthogonalize(4, 0, 0, vecs, sc, fake, fake, fake, None) == \
+        native.EMU_SV_EINVAL
+    assert lib.emu_sv_arnoldi_orthogonalize(4, 0, 1, vecs, sc, fake, fake, fake, None) == \
+        native.EMU_SV_EINVAL  # w aliases the basis vector
+    assert lib.emu_sv_sample_workspace_bytes(20) == 256 * 8
+    assert lib.emu_sv_sample(4, 0, fake, ctypes.c_uint64(1), 2, fake, 1, fake, fake,
+                             ctypes.c_size_t(8), None) == native.EMU_SV_EINVAL
+    assert lib.emu_sv_sample(20, 0, fake, ctypes.c_uint64(1), 0, fake, 1, fake, fake,
+                             ctypes.c_size_t(8), None) == native.EMU_SV_EINVAL  # workspace too small
+    assert lib.emu_sv_sample(4, 0, fake, ctypes.c_uint64(1), 0, fake, 0, fake, fake,
+                             ctypes.c_size_t(8), None) == ok  # no draws: nothing launched
+    flags = native.EMU_SV_FLAG_NOISE | native.KERNELS["tile"]
+    assert lib.emu_sv_lanczos_matvec(20, 0, fake, 0.0, -0.01, fake, 1.0, None, 1.0, 0.0, 0.0,
+                                     ctypes.c_void_p(8192), 1.0, 0.0, 0.0, ctypes.c_void_p(12288),
+                                     fake, fake, fake, 0, 0, flags, None) == native.EMU_SV_EINVAL
