@@ -954,6 +954,116 @@ __global__ __launch_bounds__(kThreads) void arnoldi_mgs_kernel(uint64_t npk, voi
   block_reduce_store<3>(acc, red, partials + (uint64_t)blockIdx.x * kNumDots);
 }
 
+// The whole modified Gram-Schmidt of a small vector (D <= 2^13) in one
+// workgroup: w stays in registers (kMgsThreads threads, D / kMgsThreads
+// elements each), every step reduces (||w||^2, <u_k|w>) over the workgroup,
+// subtracts s_k^2 <u_k|w> u_k and rounds w to the storage type, as the
+// launch chain does.  One launch instead of 2 (count + 1).
+constexpr int kMgsThreads = 1024;
+constexpr int kMgsSmallLog = 13;
+constexpr int kMgsSmallCount = 128;
+struct MgsSmallArgs {
+  const void *u[kMgsSmallCount];
+  double s2[kMgsSmallCount];
+};
+
+template <int K>
+__device__ __forceinline__ void mgs_block_sum(double (&v)[K], double *red) {
+  constexpr int NW = kMgsThreads / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double sm = wave_sum(v[k]);
+    if (lane == 0) red[wave * K + k] = sm;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double sm = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) sm += red[w * K + k];  // same order in every thread
+    v[k] = sm;
+  }
+  __syncthreads();
+}
+
+template <typename T, int EPT>
+__global__ __launch_bounds__(kMgsThreads) void arnoldi_mgs_small_kernel(int count, void *wv,
+                                                                        const MgsSmallArgs args,
+                                                                        double *records) {
+  using T2 = typename Vec2<T>::type;
+  __shared__ double red[(kMgsThreads / 64) * 3];
+  T2 *w = reinterpret_cast<T2 *>(wv);
+  double wr[EPT], wi[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const T2 v = w[threadIdx.x + e * kMgsThreads];
+    wr[e] = v.x;
+    wi[e] = v.y;
+  }
+  for (int k = 0; k < count; ++k) {
+    const T2 *u = reinterpret_cast<const T2 *>(args.u[k]);
+    double ur[EPT], ui[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const T2 q = u[threadIdx.x + e * kMgsThreads];
+      ur[e] = q.x;
+      ui[e] = q.y;
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      acc[0] += wr[e] * wr[e] + wi[e] * wi[e];
+      acc[1] += ur[e] * wr[e] + ui[e] * wi[e];
+      acc[2] += ur[e] * wi[e] - ui[e] * wr[e];
+    }
+    mgs_block_sum<3>(acc, red);
+    if (threadIdx.x == 0) {
+      records[4 * k] = acc[0];
+      records[4 * k + 1] = acc[1];
+      records[4 * k + 2] = acc[2];
+    }
+    const double hr = args.s2[k] * acc[1], hi = args.s2[k] * acc[2];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const double zr = wr[e] - (hr * ur[e] - hi * ui[e]);
+      const double zi = wi[e] - (hr * ui[e] + hi * ur[e]);
+      wr[e] = static_cast<double>(static_cast<T>(zr));
+      wi[e] = static_cast<double>(static_cast<T>(zi));
+    }
+  }
+  double nrm[1] = {0.0};
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) nrm[0] += wr[e] * wr[e] + wi[e] * wi[e];
+  mgs_block_sum<1>(nrm, red);
+  if (threadIdx.x == 0) records[4 * count] = nrm[0];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    T2 o;
+    o.x = static_cast<T>(wr[e]);
+    o.y = static_cast<T>(wi[e]);
+    w[threadIdx.x + e * kMgsThreads] = o;
+  }
+}
+
+template <typename T>
+int launch_mgs_small(int n, int count, const void *const *vecs, const double *scales, void *w,
+                     double *records, hipStream_t st) {
+  MgsSmallArgs args;
+  for (int k = 0; k < count; ++k) {
+    args.u[k] = vecs[k];
+    args.s2[k] = scales[k] * scales[k];
+  }
+  switch (n) {
+    case 10: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 1>), dim3(1), dim3(kMgsThreads), 0, st, count, w, args, records); break;
+    case 11: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 2>), dim3(1), dim3(kMgsThreads), 0, st, count, w, args, records); break;
+    case 12: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 4>), dim3(1), dim3(kMgsThreads), 0, st, count, w, args, records); break;
+    case 13: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 8>), dim3(1), dim3(kMgsThreads), 0, st, count, w, args, records); break;
+    default: return fail(EMU_SV_EINVAL, "arnoldi_orthogonalize: small path needs 10 <= n <= 13");
+  }
+  return hip_check(hipGetLastError(), "arnoldi_mgs_small launch");
+}
+
 // A 2x2 operator on one bit: thread p owns the pair (x0, x0 | 2^bit).
 struct Op1q {
   double a[8];  // A00, A01, A10, A11 as (re, im)
@@ -2365,6 +2475,10 @@ int emu_sv_arnoldi_orthogonalize(int n, int dtype, int count, const void *const 
     if (!vecs[k]) return fail(EMU_SV_EINVAL, "arnoldi_orthogonalize: null basis vector");
     if (vecs[k] == w) return fail(EMU_SV_EINVAL, "arnoldi_orthogonalize: w aliases a basis vector");
   }
+  if (n >= 10 && n <= kMgsSmallLog && count <= kMgsSmallCount)  // one workgroup, one launch
+    return dtype == EMU_SV_C128
+               ? launch_mgs_small<double>(n, count, vecs, scales, w, records, stream)
+               : launch_mgs_small<float>(n, count, vecs, scales, w, records, stream);
   const uint64_t npk = packets(n, dtype);
   const int nb = stream_blocks(npk);
   double *part = partials_ptr(workspace, n);
