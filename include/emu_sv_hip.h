@@ -207,7 +207,9 @@ int emu_sv_scale(int n, int dtype, void *v, double s, hipStream_t stream);
  * the Arnoldi branch of krylov_exp.py:52-58: for k = 0..count-1,
  * h_k = <v_k|w>, w <- w - h_k v_k.  The coefficients never leave the device:
  * launch k subtracts the previous projection (read from records_out) and
- * reduces the next overlap in the same pass over w.  records_out (device,
+ * reduces the next overlap in the same pass over w (for 10 <= n <= 13 and
+ * count <= 128, one workgroup holds w and runs the whole sequence in one
+ * launch).  records_out (device,
  * 4*(count+1) doubles): record k < count = {||w||^2 before the k-th
  * subtraction, Re, Im <V_k|w>} (unscaled: h_k = scales[k] * <V_k|w>), record
  * count = {||w||^2 after the last subtraction}.  vecs: host array of device
