@@ -69,6 +69,7 @@ SIGNATURES = {
     "emu_sv_apply_1q": (_i, [_i, _i, _vp, _i, _dp, _vp]),
     "emu_sv_apply_pair": (_i, [_i, _i, _vp, _vp, _i, _i, _dp, _i, _vp]),
     "emu_sv_antihermitian": (_i, [_i, _i, _vp, _vp, _d, _d, _vp]),
+    "emu_sv_apply_pairs_all": (_i, [_i, _i, _vp, _vp, _dp, _vp]),
     "emu_sv_sample_workspace_bytes": (ctypes.c_size_t, [_i]),
     "emu_sv_sample": (_i, [_i, _i, _vp, ctypes.c_uint64, _i, _vp, _i, _vp, _vp, ctypes.c_size_t,
                            _vp]),
@@ -388,6 +389,16 @@ def apply_pair(n: int, src: torch.Tensor, dst: torch.Tensor, bit_a: int, bit_b: 
                                 a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                 int(accumulate), _stream(src.device)), "apply_pair")
     return dst
+
+
+def apply_pairs_all(n_side: int, src: torch.Tensor, dst: torch.Tensor, op: np.ndarray) -> None:
+    """dst += the 4x4 op on every (row bit, column bit) pair of an
+    n_side-qubit density matrix (flat 4^n_side device vectors, n_side <= 8);
+    ``op`` is a contiguous complex128 (4, 4) array."""
+    a = np.ascontiguousarray(op.reshape(16)).view(np.float64)
+    check(LIB.emu_sv_apply_pairs_all(n_side, dtype_code(src.dtype), _ptr(src), _ptr(dst),
+                                      a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      _stream(src.device)), "apply_pairs_all")
 
 
 def antihermitian(m: torch.Tensor, out: torch.Tensor, s: complex = 1.0) -> torch.Tensor:

@@ -1144,6 +1144,39 @@ __global__ __launch_bounds__(kThreads) void pair_op_kernel(uint64_t ngroups, con
   }
 }
 
+// out[x] += sum_j S (row bit j+N, column bit j) of in, for every qubit j of
+// an N-qubit density matrix stored as a 2N-bit vector: the gathered form of
+// N pair operators with one S, one launch (small matrices, cache resident).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void pairs_all_kernel(int N, const void *inv, void *outv,
+                                                             const Op2q op) {
+  using T2 = typename Vec2<T>::type;
+  const T2 *in = reinterpret_cast<const T2 *>(inv);
+  T2 *out = reinterpret_cast<T2 *>(outv);
+  const uint64_t D = 1ull << (2 * N);
+  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  for (uint64_t x = (uint64_t)blockIdx.x * kThreads + threadIdx.x; x < D; x += stride) {
+    double ar = 0.0, ai = 0.0;
+    for (int j = 0; j < N; ++j) {
+      const int rb = j + N, cb = j;
+      const int row = 2 * (int)((x >> rb) & 1) + (int)((x >> cb) & 1);
+      const uint64_t base = x & ~((1ull << rb) | (1ull << cb));
+#pragma unroll
+      for (int ab = 0; ab < 4; ++ab) {
+        const uint64_t y = base | ((uint64_t)(ab >> 1) << rb) | ((uint64_t)(ab & 1) << cb);
+        const T2 v = in[y];
+        const double sr = op.s[2 * (4 * row + ab)], si = op.s[2 * (4 * row + ab) + 1];
+        ar += sr * (double)v.x - si * (double)v.y;
+        ai += sr * (double)v.y + si * (double)v.x;
+      }
+    }
+    T2 o = out[x];
+    o.x = static_cast<T>((double)o.x + ar);
+    o.y = static_cast<T>((double)o.y + ai);
+    out[x] = o;
+  }
+}
+
 // out = s (M - M^H) for a 2^k x 2^k row-major matrix M.  Block (bx, by) with
 // by <= bx owns the tile pair (I, J) = (by, bx) and (J, I): both tiles are
 // staged in LDS before either is written, so out may alias M.
@@ -2535,6 +2568,25 @@ int emu_sv_apply_pair(int n, int dtype, const void *in, void *out, int bit_a, in
     hipLaunchKernelGGL(pair_op_kernel<float>, dim3(nb), dim3(kThreads), 0, stream, ngroups, in,
                        out, bit_a, bit_b, o, accumulate ? 1 : 0);
   return hip_check(hipGetLastError(), "apply_pair launch");
+}
+
+int emu_sv_apply_pairs_all(int n_side, int dtype, const void *in, void *out, const double *op,
+                           hipStream_t stream) {
+  if (n_side < 1 || n_side > 8) return fail(EMU_SV_EINVAL, "apply_pairs_all: n_side must be in [1, 8]");
+  if (int rc = check_dtype(dtype)) return rc;
+  if (!in || !out || !op) return fail(EMU_SV_EINVAL, "apply_pairs_all: null pointer");
+  if (in == out) return fail(EMU_SV_EINVAL, "apply_pairs_all: in and out must not alias");
+  Op2q o;
+  for (int k = 0; k < 32; ++k) o.s[k] = op[k];
+  const uint64_t D = 1ull << (2 * n_side);
+  const int nb = stream_blocks(D);
+  if (dtype == EMU_SV_C128)
+    hipLaunchKernelGGL(pairs_all_kernel<double>, dim3(nb), dim3(kThreads), 0, stream, n_side, in,
+                       out, o);
+  else
+    hipLaunchKernelGGL(pairs_all_kernel<float>, dim3(nb), dim3(kThreads), 0, stream, n_side, in,
+                       out, o);
+  return hip_check(hipGetLastError(), "apply_pairs_all launch");
 }
 
 int emu_sv_antihermitian(int side_log, int dtype, const void *m, void *out, double s_re,

@@ -19,6 +19,7 @@ are the row index.  With P = H_eff rho and Q = sum_k A_k rho A_k^H,
 
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 from emulators_amd import _native as native
@@ -96,7 +97,10 @@ class RydbergLindbladian:
                     complex_coef=self.complex, **_SCHED.kwargs())
         if s_op is None:
             return
-        op = (s_op * scale).numpy()
+        op = np.ascontiguousarray((s_op * scale).numpy())
+        if n <= 8:  # every qubit's pair in one gathered launch
+            native.apply_pairs_all(n, src, dst, op)
+            return
         for q in range(n):
             native.apply_pair(2 * n, src, dst, 2 * n - 1 - q, n - 1 - q, op, accumulate=True)
 

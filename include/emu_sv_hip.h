@@ -236,6 +236,15 @@ int emu_sv_apply_1q(int n, int dtype, void *psi, int bit, const double *a,
 int emu_sv_apply_pair(int n, int dtype, const void *in, void *out, int bit_a,
                       int bit_b, const double *op, int accumulate, hipStream_t stream);
 
+/* out += sum over the qubits j < n_side of S on the bit pair (j + n_side, j)
+ * of in: the jump and noise terms of every qubit of an n_side-qubit density
+ * matrix (2*n_side-bit vector, row bits high) in one launch, in the gathered
+ * form (each output element reads its 4 n_side partners).  For small
+ * matrices, n_side <= 8; in and out must not alias.  Same terms as n_side
+ * calls of emu_sv_apply_pair (emu_sv/lindblad_operator.py:94-135, 203-213). */
+int emu_sv_apply_pairs_all(int n_side, int dtype, const void *in, void *out,
+                           const double *op, hipStream_t stream);
+
 /* out = s (M - M^H) for a 2^side_log x 2^side_log row-major complex matrix M
  * (s complex); out may alias M.  The anti-Hermitian part taken twice in
  * RydbergLindbladian.__matmul__ (emu_sv/lindblad_operator.py:196-216). */
