@@ -88,6 +88,7 @@ SIGNATURES = {
     "emu_sv_pack_hamiltonian_steps": (_i, [_i, _i, _dp, _dp, _dp, _dp, _dp]),
     "emu_sv_tridiag_eig": (_i, [_i, _dp, _dp, _dp, _dp]),
     "emu_sv_expm": (_i, [_i, _dp, _dp]),
+    "emu_sv_expm_column": (_i, [_i, _dp, _dp]),
     "emu_sv_lanczos_exp": (_i, [_i, _dp, _dp, _d, _d, _d, _i, _dp]),
 }
 
@@ -497,6 +498,19 @@ def parameter_traces(n: int, w: torch.Tensor, z: torch.Tensor, pairs: bool = Fal
         pb = o[6 * n:].view(np.complex128).reshape(n, n)  # [b][b'] for b < b'
         pair = pb[rev][:, rev].T.copy()  # qubit (i, j) = bits (n-1-i, n-1-j): i < j  <=>  b > b'
     return Traces(up, dn, occ, pair)
+
+
+def expm_column(a: np.ndarray) -> np.ndarray:
+    """expm(a)[:, 0] of a small host complex128 matrix (Taylor substeps)."""
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+    m = a.shape[0]
+    if a.shape != (m, m):
+        raise ValueError("expm_column: square matrix expected")
+    out = np.empty(m, dtype=np.complex128)
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(LIB.emu_sv_expm_column(m, a.view(np.float64).ctypes.data_as(dp),
+                                 out.view(np.float64).ctypes.data_as(dp)), "expm_column")
+    return out
 
 
 def expm(a: np.ndarray) -> np.ndarray:

@@ -136,6 +136,87 @@ extern "C" int emu_sv_expm(int m, const double *a_in, double *e_out) {
   return EMU_SV_OK;
 }
 
+// First column of expm(A): exp(A) e_0 = e^c exp(A - cI) e_0 with c the mean
+// of the diagonal, advanced in s = ceil(||A - cI||_inf / 4) substeps of a
+// 32-term Taylor series (truncation 4^33/33! < 1e-17 per substep, largest
+// term 4^4/4! ~ 11).  O(s m^2) per call instead of the O(m^3) products of the
+// Pade form; beyond ||A - cI||_inf = 256 the Pade form is used.
+extern "C" int emu_sv_expm_column(int m, const double *a_in, double *col_out) {
+  if (m < 1 || m > 4096 || !a_in || !col_out) return EMU_SV_EINVAL;
+  std::vector<cd> a((size_t)m * m);
+  cd c(0.0, 0.0);
+  for (int i = 0; i < m * m; ++i) a[i] = cd(a_in[2 * i], a_in[2 * i + 1]);
+  for (int i = 0; i < m; ++i) c += a[(size_t)i * m + i];
+  c /= (double)m;
+  for (int i = 0; i < m; ++i) a[(size_t)i * m + i] -= c;
+  double rowmax = 0.0;
+  std::vector<int> first(m), last(m);  // nonzero span of each row
+  for (int i = 0; i < m; ++i) {
+    double rs = 0.0;
+    int f = m, l = -1;
+    for (int j = 0; j < m; ++j) {
+      const cd x = a[(size_t)i * m + j];
+      if (x != cd(0.0, 0.0)) {
+        rs += std::abs(x);
+        if (f == m) f = j;
+        l = j;
+      }
+    }
+    first[i] = f;
+    last[i] = l;
+    rowmax = rs > rowmax ? rs : rowmax;
+  }
+  if (rowmax > 256.0) {
+    std::vector<double> e((size_t)2 * m * m);
+    if (int rc = emu_sv_expm(m, a_in, e.data())) return rc;
+    for (int i = 0; i < m; ++i) {
+      col_out[2 * i] = e[(size_t)2 * i * m];
+      col_out[2 * i + 1] = e[(size_t)2 * i * m + 1];
+    }
+    return EMU_SV_OK;
+  }
+  const int steps = rowmax > 0.0 ? (int)std::ceil(rowmax / 4.0) : 1;
+  const double h = 1.0 / steps;
+  // real arithmetic (std::complex products go through the library's
+  // Inf/NaN-aware multiply)
+  std::vector<double> ar((size_t)m * m), ai((size_t)m * m);
+  for (size_t i = 0; i < (size_t)m * m; ++i) {
+    ar[i] = a[i].real();
+    ai[i] = a[i].imag();
+  }
+  std::vector<double> ur(m, 0.0), ui(m, 0.0), tr(m), ti(m), yr(m), yi(m);
+  ur[0] = 1.0;
+  for (int st = 0; st < steps; ++st) {
+    tr = ur;
+    ti = ui;
+    for (int k = 1; k <= 32; ++k) {
+      const double f = h / k;
+      for (int i = 0; i < m; ++i) {
+        double sr = 0.0, si = 0.0;
+        const double *rr = &ar[(size_t)i * m], *ri = &ai[(size_t)i * m];
+        for (int j = first[i]; j <= last[i]; ++j) {
+          sr += rr[j] * tr[j] - ri[j] * ti[j];
+          si += rr[j] * ti[j] + ri[j] * tr[j];
+        }
+        yr[i] = sr * f;
+        yi[i] = si * f;
+      }
+      for (int i = 0; i < m; ++i) {
+        tr[i] = yr[i];
+        ti[i] = yi[i];
+        ur[i] += yr[i];
+        ui[i] += yi[i];
+      }
+    }
+  }
+  const double em = std::exp(c.real()), er = em * std::cos(c.imag()), ei = em * std::sin(c.imag());
+  for (int i = 0; i < m; ++i) {
+    col_out[2 * i] = er * ur[i] - ei * ui[i];
+    col_out[2 * i + 1] = er * ui[i] + ei * ur[i];
+  }
+  return EMU_SV_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Lanczos exponential from the symmetric tridiagonal form.
 //

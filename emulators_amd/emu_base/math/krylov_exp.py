@@ -33,7 +33,8 @@ Two implementations sit behind it:
   modified Gram-Schmidt of the reference (h_k = <v_k|w>, w -= h_k v_k for
   k = 0..j in order) as a chain of launches that keeps every h_k on the
   device (emu_sv_arnoldi_orthogonalize), one read-back of the j+2 records,
-  and the dense (j+3)^2 exponential of the Hessenberg matrix on the host.
+  and the first column of the (j+3)^2 Hessenberg matrix's exponential on
+  the host (the only column the estimate and the combination read).
 
 * any other callable: the same algorithm on torch tensors (Lanczos or full
   Gram-Schmidt Arnoldi), with the small exponential from the native expm.
@@ -442,12 +443,12 @@ def _fused_arnoldi(op, v: torch.Tensor, exp_tolerance: float,
         n2 = math.sqrt(max(rec[j + 1, 0], 0.0))
         T[j + 1, j] = n2
         if n2 < norm_tolerance:  # happy breakdown
-            expd = native.expm(T[: j + 1, : j + 1])
+            expd = native.expm_column(T[: j + 1, : j + 1])[:, None]
             return KrylovExpResult(combine(expd[:, 0]), True, True, j + 1)
         basis.append(w)
         scales.append(1.0 / n2)
         T[j + 2, j + 1] = 1.0
-        expd = native.expm(T[: j + 3, : j + 3])
+        expd = native.expm_column(T[: j + 3, : j + 3])[:, None]
         if _error_estimate(expd, j, nrm) < exp_tolerance:
             return KrylovExpResult(combine(expd[: len(basis), 0]), True, False, j + 1)
     return KrylovExpResult(combine(expd[: len(basis), 0]), False, False, max_krylov_dim)

@@ -378,6 +378,12 @@ int emu_sv_pack_hamiltonian_steps(int n, int steps, const double *omega, const d
  * matrix (emu_base/math/krylov_exp.py:65,78). */
 int emu_sv_expm(int m, const double *a, double *e);
 
+/* Host: col = expm(A) e_0, the first column only (m complex, interleaved),
+ * by diagonal shift and Taylor substeps of norm <= 4 (the Pade form of
+ * emu_sv_expm beyond ||A - cI||_inf = 256).  Every use of the extended
+ * Krylov matrix's exponential reads column 0 (krylov_exp.py:65-90). */
+int emu_sv_expm_column(int m, const double *a, double *col_out);
+
 /* Host: the exponential of a Hermitian Lanczos matrix from its symmetric
  * tridiagonal form.  tdiag: m complex T[k,k] (interleaved), tsub: m-1 real
  * T[k+1,k]; sigma: the operator scale (op = sigma*H); n2: the newest

@@ -422,3 +422,22 @@ def test_new_entry_points_validate_arguments_on_the_host():
     assert lib.emu_sv_lanczos_matvec(20, 0, fake, 0.0, -0.01, fake, 1.0, None, 1.0, 0.0, 0.0,
                                      ctypes.c_void_p(8192), 1.0, 0.0, 0.0, ctypes.c_void_p(12288),
                                      fake, fake, fake, 0, 0, flags, None) == native.EMU_SV_EINVAL
+
+
+@pytest.mark.parametrize("m", [1, 3, 12, 30, 62])
+@pytest.mark.parametrize("scale", [0.01, 1.0, 8.0, 40.0, 400.0])
+def test_expm_column_vs_scipy(m, scale):
+    """expm(A)[:, 0] from the Taylor-substep host routine against scipy's
+    Pade expm, on Hessenberg matrices like the Arnoldi ones (nearly
+    skew-Hermitian plus a dissipative part); beyond norm 256 it takes the
+    Pade path."""
+    import scipy.linalg as sl
+
+    from emulators_amd import _native as native
+    rng = np.random.default_rng(m * 1000 + int(scale * 10))
+    a = (rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))) * scale / np.sqrt(m)
+    a = np.triu(a, -1)
+    a = 0.5 * (a - a.conj().T) - 0.05 * abs(scale) * np.eye(m)
+    ref = sl.expm(a)[:, 0]
+    got = native.expm_column(a)
+    assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
