@@ -277,11 +277,12 @@ def lanczos_matvec(n: int, ham_dev: torch.Tensor, sigma: complex, w: torch.Tenso
 def lanczos_matvec_dev(n: int, ham_dev: torch.Tensor, sigma: complex, w: torch.Tensor,
                        pa: torch.Tensor | None, pb: torch.Tensor, coef: torch.Tensor, r: torch.Tensor,
                        rec: torch.Tensor, complex_coef: bool = True, tile_log: int = 0,
-                       min_contig: int = 0, partners: int = 0, first_log: int = 0) -> None:
+                       min_contig: int = 0, partners: int = 0, first_log: int = 0,
+                       kernel: str = "tile") -> None:
     """lanczos_matvec with device coefficients coef (6 doubles: a*pa_scale,
-    b*pb_scale, pb_scale, w_scale) for the tile kernel; rec (device, >= 12
-    doubles) receives [||z||^2, <z|pb_s>, <z|w_s>, ||r||^2, <z|r>, ...]."""
-    flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags("tile", partners, first_log)
+    b*pb_scale, pb_scale, w_scale) for the tile or LDS kernel; rec (device,
+    >= 12 doubles) receives [||z||^2, <z|pb_s>, <z|w_s>, ||r||^2, <z|r>, ...]."""
+    flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags(kernel, partners, first_log)
     check(LIB.emu_sv_lanczos_matvec_dev(n, dtype_code(w.dtype), _ptr(ham_dev), float(sigma.real),
                                         float(sigma.imag), _ptr(w), _ptr(pa), _ptr(pb), _ptr(coef),
                                         _ptr(r), _ptr(rec[0:5]), _ptr(rec[5:12]), _ptr(

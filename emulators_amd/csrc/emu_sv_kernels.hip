@@ -113,6 +113,10 @@ struct PassArgs {
   double ca_re, ca_im, cb_re, cb_im;
   double pb_scale;
   double *partials2;
+  // Orthogonalising pass of a pipelined Lanczos iteration: when set, ca_re,
+  // ca_im, cb_re, cb_im, pb_scale and v_scale are read from these 6 device
+  // doubles instead (formed on the device by emu_sv_lanczos_next_coef).
+  const double *dcoef;
   double v_scale, prev_scale;
   double sig_re, sig_im;
   int n;               // qubits
@@ -472,7 +476,16 @@ __device__ __forceinline__ void pass_tile(const PassArgs &a, uint64_t g, const d
 }
 
 template <typename T, bool CPLX, bool DOTS, bool NOISE = false>
-__global__ __launch_bounds__(kThreads) void hpsi_pass_kernel(const PassArgs a) {
+__global__ __launch_bounds__(kThreads) void hpsi_pass_kernel(const PassArgs args) {
+  PassArgs a = args;
+  if (args.dcoef) {
+    a.ca_re = args.dcoef[0];
+    a.ca_im = args.dcoef[1];
+    a.cb_re = args.dcoef[2];
+    a.cb_im = args.dcoef[3];
+    a.pb_scale = args.dcoef[4];
+    a.v_scale = args.dcoef[5];
+  }
   const int t = a.c + a.m;
   const uint64_t ntiles = 1ull << (a.n - t);
   uint64_t g = blockIdx.x;
@@ -1751,7 +1764,7 @@ struct OrthoIn {
   const void *pa, *pb;
   double ca_re, ca_im, cb_re, cb_im, pb_scale;
   double *norm2_out;  // kOrthoSums doubles
-  const double *dcoef = nullptr;  // device coefficients (tile kernel only)
+  const double *dcoef = nullptr;  // device coefficients (LDS and tile kernels)
 };
 
 template <typename T>
@@ -1771,8 +1784,10 @@ int run_hpsi(int n, const double *ham, double sr, double si, const void *v, doub
                        const_cast<double *>(dtab_ptr(ws, n)));
     if (int rc = hip_check(hipGetLastError(), "diag_prep launch")) return rc;
   }
+  if (ortho && ortho->dcoef && (kind == kKernelWave || noise))
+    return fail(EMU_SV_EINVAL, "hpsi: device coefficients need the LDS or tile kernel");
   for (int p = 0; p < np; ++p) {
-    PassArgs a;
+    PassArgs a{};
     a.v = v;
     a.prev = prev;
     a.out = w;
@@ -1789,6 +1804,7 @@ int run_hpsi(int n, const double *ham, double sr, double si, const void *v, doub
     a.cb_im = ortho ? ortho->cb_im : 0.0;
     a.pb_scale = ortho ? ortho->pb_scale : 0.0;
     a.partials2 = partials2_ptr(ws, n);
+    a.dcoef = (ortho != nullptr && p == 0) ? ortho->dcoef : nullptr;
     a.v_scale = (ortho != nullptr && p > 0) ? 1.0 : vsc;
     a.prev_scale = psc;
     a.sig_re = sr;
@@ -2392,14 +2408,23 @@ int emu_sv_lanczos_matvec_dev(int n, int dtype, const double *ham, double sigma_
     return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: null pointer");
   if (w == r || w == pa || w == pb || r == pa || r == pb)
     return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: w, r, pa and pb must be distinct buffers");
-  if (kernel_kind(flags) != kKernelTile)
-    return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: needs the tile kernel kind");
+  const int kind = kernel_kind(flags);
+  if (kind != kKernelTile && kind != kKernelLds)
+    return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: needs the tile or LDS kernel kind");
   if (flags & EMU_SV_FLAG_NOISE)
     return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: the Lanczos recurrence needs a Hermitian H (no noise term)");
-  int tla = tile_first_log(flags, 0);
-  const int rc = check_tile_schedule(n, dtype, tile_log, min_contig, tla, "lanczos_matvec_dev");
-  if (rc < 0) return rc;
-  if (rc > 0) return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: fewer qubits than one tile");
+  int tla = 0;
+  if (kind == kKernelTile) {
+    tla = tile_first_log(flags, 0);
+    const int rc = check_tile_schedule(n, dtype, tile_log, min_contig, tla, "lanczos_matvec_dev");
+    if (rc < 0) return rc;
+    if (rc > 0) return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: fewer qubits than one tile");
+  } else {
+    default_tiles(dtype, kind, tile_log, min_contig);
+    if (tile_log < kThreadBits || tile_log > kMaxTileLog || (dtype == EMU_SV_C128 && tile_log > 13) ||
+        min_contig < 0 || min_contig >= tile_log)
+      return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: bad tile_log / min_contig");
+  }
   OrthoIn o;
   o.pa = pa;
   o.pb = pb;
@@ -2408,6 +2433,13 @@ int emu_sv_lanczos_matvec_dev(int n, int dtype, const double *ham, double sigma_
   o.norm2_out = norm2_out;
   o.dcoef = coef;
   const bool cplx = (flags & EMU_SV_FLAG_COMPLEX) != 0;
+  if (kind == kKernelLds) {
+    if (dtype == EMU_SV_C128)
+      return run_hpsi<double>(n, ham, sigma_re, sigma_im, w, 1.0, nullptr, 0.0, r, dots_out, workspace,
+                              kind, tile_log, min_contig, cplx, stream, &o);
+    return run_hpsi<float>(n, ham, sigma_re, sigma_im, w, 1.0, nullptr, 0.0, r, dots_out, workspace,
+                           kind, tile_log, min_contig, cplx, stream, &o);
+  }
   if (dtype == EMU_SV_C128)
     return run_tile<double>(n, ham, sigma_re, sigma_im, w, 1.0, nullptr, 0.0, r, dots_out, workspace,
                             tla, tile_log, min_contig, tile_partners(flags), cplx, stream, &o);

@@ -294,8 +294,8 @@ def _pipelined_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: fl
     dev, host = buf.dev, buf.host
     stream = torch.cuda.current_stream(v.device)
     sig = op.sigma
-    kw = dict(tile_log=sched.tile_log, min_contig=sched.min_contig, partners=sched.partners,
-              first_log=sched.first_log)
+    kw = dict(kernel=sched.kernel, tile_log=sched.tile_log, min_contig=sched.min_contig,
+              partners=sched.partners, first_log=sched.first_log)
 
     initial_norm = math.sqrt(native.dot(n, v)[0].item())
     basis = [v]
@@ -310,7 +310,7 @@ def _pipelined_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: fl
 
     wbuf = new_vec(1)
     d = native.hpsi(n, params, sig, v, scales[0], wbuf, dots=True, complex_coef=ham.complex,
-                    kernel="tile", **kw).cpu().numpy()
+                    **kw).cpu().numpy()
     KrylovStats.matvecs += 1
     b0 = complex(d[1], d[2])
     T[0, 0] = b0
@@ -385,15 +385,15 @@ def fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
                   norm_tolerance: float, max_krylov_dim: int,
                   rows: torch.Tensor | None = None) -> LanczosRun:
     """The fused engine: pipelined (device-side coefficients) when the tuned
-    schedule is the tile kernel and N covers its tiles, else one read-back
-    per iteration.  EMU_SV_PIPELINE=0 turns the pipeline off."""
+    schedule is the tile or LDS kernel and N covers its tiles, else one
+    read-back per iteration.  EMU_SV_PIPELINE=0 turns the pipeline off."""
     ham = op.hamiltonian
     n = native.nqubits_of(v)
     if n == ham.nqubits:
         sched = autotune.best_schedule(n, v.dtype, v.device, ham.device_params(v.device), ham.complex)
         tl = sched.tile_log or (13 if v.dtype == torch.complex64 else 12)
         fl = sched.first_log or tl
-        if PIPELINE and sched.kernel == "tile" and n >= max(tl, fl):
+        if PIPELINE and sched.kernel in ("tile", "lds") and n >= max(tl, fl):
             return _pipelined_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim, sched,
                                       rows)
     return _fused_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim, rows)

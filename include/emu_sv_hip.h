@@ -158,9 +158,9 @@ int emu_sv_lanczos_matvec(int n, int dtype, const double *ham, double sigma_re,
 /* emu_sv_lanczos_matvec with its coefficients in device memory, so that a
  * caller can enqueue iteration j+1 before reading iteration j's results:
  * coef = {Re, Im of a*pa_scale, Re, Im of b*pb_scale, pb_scale, w_scale}
- * (6 doubles, device), as written by emu_sv_lanczos_next_coef.  Tile kernel
- * kind only (flags), n >= the tile sizes.  Same outputs as
- * emu_sv_lanczos_matvec (krylov_exp.py:50-61). */
+ * (6 doubles, device), as written by emu_sv_lanczos_next_coef.  Tile or LDS
+ * kernel kind (flags; the wave kind is rejected), n >= the tile sizes.  Same
+ * outputs as emu_sv_lanczos_matvec (krylov_exp.py:50-61). */
 int emu_sv_lanczos_matvec_dev(int n, int dtype, const double *ham, double sigma_re,
                               double sigma_im, void *w, const void *pa, const void *pb,
                               const double *coef, void *r, double *norm2_out,

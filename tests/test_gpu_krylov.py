@@ -251,23 +251,34 @@ def _dense_h_autograd(om, de, ph, u, n):
     return h
 
 
+_PIPE_SCHEDULES = {
+    "tile": "c128=tile:11:4:0:11,c64=tile:11:4:0:11",
+    "lds": "c128=lds:11:2,c64=lds:11:3",
+    "lds10": "c128=lds:10:2,c64=lds:10:3",
+}
+
+
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
 @pytest.mark.parametrize("n", [11, 14])
-def test_pipelined_engine_matches_unpipelined(gpu_device, monkeypatch, dtype, n):
+@pytest.mark.parametrize("kernel", ["tile", "lds", "lds10"])
+@pytest.mark.parametrize("phase", [True, False])
+def test_pipelined_engine_matches_unpipelined(gpu_device, monkeypatch, dtype, n, kernel, phase):
     """The pipelined engine (device-side coefficients, host one iteration
-    behind) against the one-read-back-per-iteration engine on the same tile
-    schedule: same Lanczos iteration count, same result, and the oracle."""
+    behind) against the one-read-back-per-iteration engine on the same
+    schedule (tile kernel, LDS kernel with 2^11 and 2^10 tiles; complex and
+    real flip coefficients): same Lanczos iteration count, same result, and
+    the oracle."""
     import importlib
 
     from emulators_amd import autotune
     from emulators_amd.emu_sv import RydbergHamiltonian
 
     kx = importlib.import_module("emulators_amd.emu_base.math.krylov_exp")
-    monkeypatch.setenv("EMU_SV_SCHEDULE", "c128=tile:11:4:0:11,c64=tile:11:4:0:11")
+    monkeypatch.setenv("EMU_SV_SCHEDULE", _PIPE_SCHEDULES[kernel])
     g = torch.Generator().manual_seed(40 + n)
     om = torch.rand(n, generator=g, dtype=torch.float64) * 10
     de = (torch.rand(n, generator=g, dtype=torch.float64) - 0.5) * 10
-    ph = torch.rand(n, generator=g, dtype=torch.float64) * 3
+    ph = torch.rand(n, generator=g, dtype=torch.float64) * 3 * phase
     u = torch.rand(n, n, generator=g, dtype=torch.float64) * 20
     u = u + u.T
     psi = torch.randn(2**n, dtype=torch.complex128, generator=g)
@@ -275,11 +286,11 @@ def test_pipelined_engine_matches_unpipelined(gpu_device, monkeypatch, dtype, n)
     ham = RydbergHamiltonian(om, de, ph, u, device=gpu_device)
     op = kx.ScaledHamiltonian(ham, -0.01j)
     tol = 1e-10
-    assert autotune.best_schedule(n, dtype, gpu_device, ham.device_params(gpu_device), True).kernel == "tile"
+    sched = autotune.best_schedule(n, dtype, gpu_device, ham.device_params(gpu_device), ham.complex)
+    assert sched.kernel == kernel.rstrip("0123456789")
+    assert ham.complex == phase
     v = psi.to(gpu_device, dtype)
-    piped = kx._pipelined_lanczos(op, v.clone(), tol, tol, 100,
-                                  autotune.best_schedule(n, dtype, gpu_device,
-                                                         ham.device_params(gpu_device), True))
+    piped = kx._pipelined_lanczos(op, v.clone(), tol, tol, 100, sched)
     plain = kx._fused_lanczos(op, v.clone(), tol, tol, 100)
     assert piped.converged and plain.converged
     assert piped.iteration_count == plain.iteration_count
@@ -291,5 +302,9 @@ def test_pipelined_engine_matches_unpipelined(gpu_device, monkeypatch, dtype, n)
                          tol, tol)
     assert float((a - ref).norm()) < (1e-9 if dtype == torch.complex128 else 1e-5)
     # and through krylov_exp (the dispatcher picks the pipelined engine)
+    calls = []
+    real = kx._pipelined_lanczos
+    monkeypatch.setattr(kx, "_pipelined_lanczos", lambda *a, **k: calls.append(1) or real(*a, **k))
     out = kx.krylov_exp(op, v.clone(), tol, tol)
+    assert calls, "krylov_exp did not take the pipelined engine"
     assert float((out.cpu().to(torch.complex128) - a).norm()) < tolv

@@ -422,6 +422,16 @@ def test_new_entry_points_validate_arguments_on_the_host():
     assert lib.emu_sv_lanczos_matvec(20, 0, fake, 0.0, -0.01, fake, 1.0, None, 1.0, 0.0, 0.0,
                                      ctypes.c_void_p(8192), 1.0, 0.0, 0.0, ctypes.c_void_p(12288),
                                      fake, fake, fake, 0, 0, flags, None) == native.EMU_SV_EINVAL
+    # device coefficients: the wave kernel (and the default kind) cannot read them
+    for kind in ("wave", "default"):
+        assert lib.emu_sv_lanczos_matvec_dev(20, 0, fake, 0.0, -0.01, ctypes.c_void_p(8192), None,
+                                             ctypes.c_void_p(12288), fake, ctypes.c_void_p(16384),
+                                             fake, fake, fake, 0, 0, native.KERNELS[kind],
+                                             None) == native.EMU_SV_EINVAL
+    assert lib.emu_sv_lanczos_matvec_dev(20, 0, fake, 0.0, -0.01, ctypes.c_void_p(8192), None,
+                                         ctypes.c_void_p(12288), fake, ctypes.c_void_p(16384),
+                                         fake, fake, fake, 14, 2, native.KERNELS["lds"],
+                                         None) == native.EMU_SV_EINVAL  # c128 LDS tiles <= 2^13
 
 
 @pytest.mark.parametrize("m", [1, 3, 12, 30, 62])
