@@ -24,6 +24,7 @@ EMU_SV_C128 = 0
 EMU_SV_C64 = 1
 EMU_SV_FLAG_COMPLEX = 1
 EMU_SV_FLAG_NOISE = 2
+EMU_SV_FLAG_KEEP_DIAG = 4
 EMU_SV_KERNEL_DEFAULT = 0
 EMU_SV_KERNEL_LDS = 1 << 8
 EMU_SV_KERNEL_WAVE = 2 << 8
@@ -229,9 +230,11 @@ def hpsi(n: int, ham_dev: torch.Tensor, sigma: complex, v: torch.Tensor, v_scale
          w: torch.Tensor, prev: torch.Tensor | None = None, prev_scale: float = 1.0,
          dots: bool = False, complex_coef: bool = True, tile_log: int = 0,
          min_contig: int = 0, kernel: str = "default", partners: int = 0,
-         first_log: int = 0, noise: bool = False) -> torch.Tensor | None:
+         first_log: int = 0, noise: bool = False, keep_diag: bool = False) -> torch.Tensor | None:
     """w = sigma * H (v_scale v); optionally returns the 7 fused dots (device).
-    ``noise``: the block carries the non-Hermitian term (LDS kernel)."""
+    ``noise``: the block carries the non-Hermitian term (LDS kernel).
+    ``keep_diag``: reuse the workspace's diagonal table when it was built for
+    this block, n and first-pass tile (EMU_SV_FLAG_KEEP_DIAG)."""
     _require_device_vec(v, "hpsi")
     _require_device_vec(w, "hpsi")
     if v.dtype != w.dtype or v.numel() != w.numel() or v.numel() != (1 << n):
@@ -243,6 +246,8 @@ def hpsi(n: int, ham_dev: torch.Tensor, sigma: complex, v: torch.Tensor, v_scale
     flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags(kernel, partners, first_log)
     if noise:
         flags |= EMU_SV_FLAG_NOISE
+    if keep_diag:
+        flags |= EMU_SV_FLAG_KEEP_DIAG
     check(LIB.emu_sv_hpsi(n, dtype_code(v.dtype), _ptr(ham_dev), float(sigma.real),
                           float(sigma.imag), _ptr(v), float(v_scale), _ptr(prev),
                           float(prev_scale), _ptr(w), _ptr(out), _ptr(ws.scratch), tile_log,
@@ -254,10 +259,11 @@ def lanczos_matvec(n: int, ham_dev: torch.Tensor, sigma: complex, w: torch.Tenso
                    pa: torch.Tensor | None, pa_scale: float, a: complex, pb: torch.Tensor,
                    pb_scale: float, b: complex, r: torch.Tensor, complex_coef: bool = True,
                    tile_log: int = 0, min_contig: int = 0, kernel: str = "default",
-                   partners: int = 0, first_log: int = 0) -> torch.Tensor:
+                   partners: int = 0, first_log: int = 0, keep_diag: bool = False) -> torch.Tensor:
     """z = w_scale w - a pa_scale pa - b pb_scale pb (into w); r = sigma H z.
     Returns a device slice [||z||^2, Re/Im <z|pb_s>, Re/Im <z|w_s>, ||r||^2,
-    Re/Im <z|r>] (pb_s = pb_scale pb, w_s = w_scale w)."""
+    Re/Im <z|r>] (pb_s = pb_scale pb, w_s = w_scale w).  ``keep_diag``: as
+    for hpsi."""
     for t in (w, pb, r) + ((pa,) if pa is not None else ()):
         _require_device_vec(t, "lanczos_matvec")
         if t.dtype != w.dtype or t.numel() != (1 << n):
@@ -265,6 +271,8 @@ def lanczos_matvec(n: int, ham_dev: torch.Tensor, sigma: complex, w: torch.Tenso
     ws = Workspace.get(w.device, n)
     out = ws.results[24:36]
     flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags(kernel, partners, first_log)
+    if keep_diag:
+        flags |= EMU_SV_FLAG_KEEP_DIAG
     check(LIB.emu_sv_lanczos_matvec(n, dtype_code(w.dtype), _ptr(ham_dev), float(sigma.real),
                                     float(sigma.imag), _ptr(w), float(w_scale), _ptr(pa),
                                     float(pa_scale), float(a.real), float(a.imag), _ptr(pb),
@@ -278,11 +286,14 @@ def lanczos_matvec_dev(n: int, ham_dev: torch.Tensor, sigma: complex, w: torch.T
                        pa: torch.Tensor | None, pb: torch.Tensor, coef: torch.Tensor, r: torch.Tensor,
                        rec: torch.Tensor, complex_coef: bool = True, tile_log: int = 0,
                        min_contig: int = 0, partners: int = 0, first_log: int = 0,
-                       kernel: str = "tile") -> None:
+                       kernel: str = "tile", keep_diag: bool = False) -> None:
     """lanczos_matvec with device coefficients coef (6 doubles: a*pa_scale,
     b*pb_scale, pb_scale, w_scale) for the tile or LDS kernel; rec (device,
-    >= 12 doubles) receives [||z||^2, <z|pb_s>, <z|w_s>, ||r||^2, <z|r>, ...]."""
+    >= 12 doubles) receives [||z||^2, <z|pb_s>, <z|w_s>, ||r||^2, <z|r>, ...].
+    ``keep_diag``: as for hpsi."""
     flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags(kernel, partners, first_log)
+    if keep_diag:
+        flags |= EMU_SV_FLAG_KEEP_DIAG
     check(LIB.emu_sv_lanczos_matvec_dev(n, dtype_code(w.dtype), _ptr(ham_dev), float(sigma.real),
                                         float(sigma.imag), _ptr(w), _ptr(pa), _ptr(pb), _ptr(coef),
                                         _ptr(r), _ptr(rec[0:5]), _ptr(rec[5:12]), _ptr(

@@ -28,6 +28,7 @@
 #include <complex>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "emu_sv_hip.h"
@@ -57,6 +58,7 @@ constexpr int kMaxTileLog = 14;   // LDS tile <= 2^14 elements
 constexpr int kThreadBits = 8;    // log2(kThreads)
 constexpr int kNumDots = 8;
 constexpr int kDiagRow = 16;  // E_out, a_0 .. a_{t-1} of one pass-1 tile
+static_assert(kDiagRow >= kMaxTileLog + 1, "a diagonal row holds E_out and one a_p per tile bit");
 constexpr int kStreamBlocks = 2048;  // grid of the streaming kernels
 constexpr int kDefaultTileLogC64 = 13;
 constexpr int kDefaultTileLogC128 = 12;
@@ -705,36 +707,30 @@ __global__ __launch_bounds__(kThreads) void hpsi_wave_kernel(const PassArgs a) {
 // Per-tile constants of the diagonal for a pass-1 tile [0, t): with h the
 // outer bits (x >> t), E_out(h) = sum_{q in h} -d_q + sum_{q<q' in h} U_qq'
 // and a_p(h) = -d_p + sum_{q in h} U_pq for the t tile bits.  One thread per
-// tile; every load is independent, so the loops pipeline.
+// table entry (row h, entry p): entry 0 is E_out(h), entry 1 + p is a_p(h),
+// entries beyond t are zero.  The table is written coalesced.
 __global__ __launch_bounds__(kThreads) void diag_prep_kernel(int n, int t, const double *H,
                                                              double *table) {
   const uint64_t ntiles = 1ull << (n - t);
-  const uint64_t h = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  const uint64_t idx = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  const uint64_t h = idx / kDiagRow;
+  const int p = (int)(idx % kDiagRow);
   if (h >= ntiles) return;
   const uint64_t outer = h << t;
-  double row[kDiagRow];
-#pragma unroll
-  for (int p = 0; p < kDiagRow; ++p) row[p] = 0.0;
   double e = 0.0;
-  for (int q = t; q < n; ++q) {
-    if (!((outer >> q) & 1)) continue;
-    e -= H[kDelta + q];
-    for (int q2 = q + 1; q2 < n; ++q2)
-      if ((outer >> q2) & 1) e += H[kU + q * 32 + q2];
-  }
-  row[0] = e;
-#pragma unroll
-  for (int p = 0; p < kDiagRow - 1; ++p) {
-    if (p < t) {
-      double ap = -H[kDelta + p];
-      for (int q = t; q < n; ++q)
-        if ((outer >> q) & 1) ap += H[kU + p * 32 + q];
-      row[1 + p] = ap;
+  if (p == 0) {
+    for (int q = t; q < n; ++q) {
+      if (!((outer >> q) & 1)) continue;
+      e -= H[kDelta + q];
+      for (int q2 = q + 1; q2 < n; ++q2)
+        if ((outer >> q2) & 1) e += H[kU + q * 32 + q2];
     }
+  } else if (p <= t) {
+    e = -H[kDelta + p - 1];
+    for (int q = t; q < n; ++q)
+      if ((outer >> q) & 1) e += H[kU + (p - 1) * 32 + q];
   }
-  double *out = table + h * kDiagRow;
-#pragma unroll
-  for (int p = 0; p < kDiagRow; ++p) out[p] = row[p];
+  table[idx] = e;
 }
 
 // Two-stage deterministic reduction of nrow rows of kNumDots partial sums:
@@ -1767,25 +1763,51 @@ struct OrthoIn {
   const double *dcoef = nullptr;  // device coefficients (LDS and tile kernels)
 };
 
+// The diagonal table of the first pass lives in the workspace.  Each
+// workspace remembers (host side) which (parameter block, n, first-pass tile
+// log) its table was last built for; with keep (EMU_SV_FLAG_KEEP_DIAG) a
+// matching table is reused instead of rebuilt.  Different blocks, sizes or
+// tiles always rebuild.
+struct DiagRecord {
+  const double *ham;
+  int n, t1;
+};
+std::mutex g_diag_mu;
+std::unordered_map<const void *, DiagRecord> g_diag_tables;
+
+int prepare_diag(int n, int t1, const double *ham, void *ws, bool keep, hipStream_t st) {
+  {
+    std::lock_guard<std::mutex> lock(g_diag_mu);
+    auto it = g_diag_tables.find(ws);
+    if (keep && it != g_diag_tables.end() && it->second.ham == ham && it->second.n == n &&
+        it->second.t1 == t1)
+      return EMU_SV_OK;
+    g_diag_tables[ws] = DiagRecord{ham, n, t1};
+  }
+  const uint64_t ntiles = 1ull << (n - t1);
+  hipLaunchKernelGGL(diag_prep_kernel, dim3((unsigned)((ntiles * kDiagRow + kThreads - 1) / kThreads)),
+                     dim3(kThreads), 0, st, n, t1, ham, const_cast<double *>(dtab_ptr(ws, n)));
+  const int rc = hip_check(hipGetLastError(), "diag_prep launch");
+  if (rc) {
+    std::lock_guard<std::mutex> lock(g_diag_mu);
+    g_diag_tables.erase(ws);
+  }
+  return rc;
+}
+
 template <typename T>
 int run_hpsi(int n, const double *ham, double sr, double si, const void *v, double vsc,
              const void *prev, double psc, void *w, double *dots, void *ws, int kind, int tl,
              int cmin, bool cplx, hipStream_t st, const OrthoIn *ortho = nullptr,
-             bool noise = false) {
+             bool noise = false, bool keep_diag = false) {
   if (kind == kKernelWave && n < tl) kind = kKernelLds;  // one small tile: LDS kernel
   if (noise && kind != kKernelLds) return fail(EMU_SV_EINVAL, "hpsi: the noise term needs the LDS kernel");
-  Pass plan[EMU_SV_MAX_QUBITS];
-  const int np = make_plan(n, tl, cmin, plan);
-  {
-    const int t1 = plan[0].c + plan[0].m;  // pass 1 is the contiguous tile [0, t1)
-    const uint64_t ntiles = 1ull << (n - t1);
-    hipLaunchKernelGGL(diag_prep_kernel, dim3((unsigned)((ntiles + kThreads - 1) / kThreads)),
-                       dim3(kThreads), 0, st, n, t1, ham,
-                       const_cast<double *>(dtab_ptr(ws, n)));
-    if (int rc = hip_check(hipGetLastError(), "diag_prep launch")) return rc;
-  }
   if (ortho && ortho->dcoef && (kind == kKernelWave || noise))
     return fail(EMU_SV_EINVAL, "hpsi: device coefficients need the LDS or tile kernel");
+  Pass plan[EMU_SV_MAX_QUBITS];
+  const int np = make_plan(n, tl, cmin, plan);
+  // pass 1 is the contiguous tile [0, t1)
+  if (int rc = prepare_diag(n, plan[0].c + plan[0].m, ham, ws, keep_diag, st)) return rc;
   for (int p = 0; p < np; ++p) {
     PassArgs a{};
     a.v = v;
@@ -1950,15 +1972,11 @@ int launch_tile(const TileArgs &a, int tl, bool cplx, bool modeb, hipStream_t st
 template <typename T>
 int run_tile(int n, const double *ham, double sr, double si, const void *v, double vsc,
              const void *prev, double psc, void *w, double *dots, void *ws, int tla, int tlb,
-             int cmin, int max_partner, bool cplx, hipStream_t st, const OrthoIn *ortho) {
+             int cmin, int max_partner, bool cplx, hipStream_t st, const OrthoIn *ortho,
+             bool keep_diag = false) {
   TilePass plan[EMU_SV_MAX_QUBITS];
   const int np = make_tile_plan(n, tla, tlb, cmin, max_partner, plan);
-  {
-    const uint64_t ntiles = 1ull << (n - tla);
-    hipLaunchKernelGGL(diag_prep_kernel, dim3((unsigned)((ntiles + kThreads - 1) / kThreads)),
-                       dim3(kThreads), 0, st, n, tla, ham, const_cast<double *>(dtab_ptr(ws, n)));
-    if (int rc = hip_check(hipGetLastError(), "diag_prep launch")) return rc;
-  }
+  if (int rc = prepare_diag(n, tla, ham, ws, keep_diag, st)) return rc;
   for (int p = 0; p < np; ++p) {
     const bool last = p == np - 1;
     TileArgs a{};
@@ -2290,12 +2308,13 @@ int emu_sv_hpsi(int n, int dtype, const double *ham, double sigma_re, double sig
   if (int rc = check_dtype(dtype)) return rc;
   if (!ham || !v || !w) return fail(EMU_SV_EINVAL, "hpsi: null pointer");
   if (v == w) return fail(EMU_SV_EINVAL, "hpsi: input and output must not alias");
-  if (dots_out && !workspace) return fail(EMU_SV_EINVAL, "hpsi: reductions need a workspace");
+  if (!workspace) return fail(EMU_SV_EINVAL, "hpsi: null workspace (it holds the diagonal table)");
   int kind = kernel_kind(flags);
   if (kind != kKernelLds && kind != kKernelWave && kind != kKernelTile)
     return fail(EMU_SV_EINVAL, "hpsi: unknown kernel kind in flags");
   const bool noise = (flags & EMU_SV_FLAG_NOISE) != 0;
   const bool cplx = noise || (flags & EMU_SV_FLAG_COMPLEX) != 0;
+  const bool keep = (flags & EMU_SV_FLAG_KEEP_DIAG) != 0;
   if (noise && kind != kKernelLds) {  // only the LDS kernel carries the noise term
     kind = kKernelLds;
     tile_log = 0;
@@ -2313,11 +2332,11 @@ int emu_sv_hpsi(int n, int dtype, const double *ham, double sigma_re, double sig
     } else if (dtype == EMU_SV_C128) {
       return run_tile<double>(n, ham, sigma_re, sigma_im, v, v_scale, prev, prev_scale, w, dots_out,
                               workspace, tla, tile_log, min_contig, tile_partners(flags), cplx,
-                              stream, nullptr);
+                              stream, nullptr, keep);
     } else {
       return run_tile<float>(n, ham, sigma_re, sigma_im, v, v_scale, prev, prev_scale, w, dots_out,
                              workspace, tla, tile_log, min_contig, tile_partners(flags), cplx,
-                             stream, nullptr);
+                             stream, nullptr, keep);
     }
   }
   default_tiles(dtype, kind, tile_log, min_contig);
@@ -2331,9 +2350,9 @@ int emu_sv_hpsi(int n, int dtype, const double *ham, double sigma_re, double sig
     return fail(EMU_SV_EINVAL, "hpsi: min_contig must be in [0, tile_log)");
   if (dtype == EMU_SV_C128)
     return run_hpsi<double>(n, ham, sigma_re, sigma_im, v, v_scale, prev, prev_scale, w, dots_out,
-                            workspace, kind, tile_log, min_contig, cplx, stream, nullptr, noise);
+                            workspace, kind, tile_log, min_contig, cplx, stream, nullptr, noise, keep);
   return run_hpsi<float>(n, ham, sigma_re, sigma_im, v, v_scale, prev, prev_scale, w, dots_out,
-                         workspace, kind, tile_log, min_contig, cplx, stream, nullptr, noise);
+                         workspace, kind, tile_log, min_contig, cplx, stream, nullptr, noise, keep);
 }
 
 int emu_sv_lanczos_matvec(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
@@ -2382,20 +2401,21 @@ int emu_sv_lanczos_matvec(int n, int dtype, const double *ham, double sigma_re, 
   o.pb_scale = pb_scale;
   o.norm2_out = norm2_out;
   const bool cplx = (flags & EMU_SV_FLAG_COMPLEX) != 0;
+  const bool keep = (flags & EMU_SV_FLAG_KEEP_DIAG) != 0;
   if (use_tile) {
     if (dtype == EMU_SV_C128)
       return run_tile<double>(n, ham, sigma_re, sigma_im, w, w_scale, nullptr, 0.0, r, dots_out,
                               workspace, tla, tile_log, min_contig, tile_partners(flags), cplx,
-                              stream, &o);
+                              stream, &o, keep);
     return run_tile<float>(n, ham, sigma_re, sigma_im, w, w_scale, nullptr, 0.0, r, dots_out,
                            workspace, tla, tile_log, min_contig, tile_partners(flags), cplx, stream,
-                           &o);
+                           &o, keep);
   }
   if (dtype == EMU_SV_C128)
     return run_hpsi<double>(n, ham, sigma_re, sigma_im, w, w_scale, nullptr, 0.0, r, dots_out,
-                            workspace, kind, tile_log, min_contig, cplx, stream, &o);
+                            workspace, kind, tile_log, min_contig, cplx, stream, &o, false, keep);
   return run_hpsi<float>(n, ham, sigma_re, sigma_im, w, w_scale, nullptr, 0.0, r, dots_out,
-                         workspace, kind, tile_log, min_contig, cplx, stream, &o);
+                         workspace, kind, tile_log, min_contig, cplx, stream, &o, false, keep);
 }
 
 int emu_sv_lanczos_matvec_dev(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
@@ -2433,18 +2453,19 @@ int emu_sv_lanczos_matvec_dev(int n, int dtype, const double *ham, double sigma_
   o.norm2_out = norm2_out;
   o.dcoef = coef;
   const bool cplx = (flags & EMU_SV_FLAG_COMPLEX) != 0;
+  const bool keep = (flags & EMU_SV_FLAG_KEEP_DIAG) != 0;
   if (kind == kKernelLds) {
     if (dtype == EMU_SV_C128)
       return run_hpsi<double>(n, ham, sigma_re, sigma_im, w, 1.0, nullptr, 0.0, r, dots_out, workspace,
-                              kind, tile_log, min_contig, cplx, stream, &o);
+                              kind, tile_log, min_contig, cplx, stream, &o, false, keep);
     return run_hpsi<float>(n, ham, sigma_re, sigma_im, w, 1.0, nullptr, 0.0, r, dots_out, workspace,
-                           kind, tile_log, min_contig, cplx, stream, &o);
+                           kind, tile_log, min_contig, cplx, stream, &o, false, keep);
   }
   if (dtype == EMU_SV_C128)
     return run_tile<double>(n, ham, sigma_re, sigma_im, w, 1.0, nullptr, 0.0, r, dots_out, workspace,
-                            tla, tile_log, min_contig, tile_partners(flags), cplx, stream, &o);
+                            tla, tile_log, min_contig, tile_partners(flags), cplx, stream, &o, keep);
   return run_tile<float>(n, ham, sigma_re, sigma_im, w, 1.0, nullptr, 0.0, r, dots_out, workspace, tla,
-                         tile_log, min_contig, tile_partners(flags), cplx, stream, &o);
+                         tile_log, min_contig, tile_partners(flags), cplx, stream, &o, keep);
 }
 
 int emu_sv_lanczos_next_coef(const double *norm2, const double *dots, double sigma_re, double sigma_im,

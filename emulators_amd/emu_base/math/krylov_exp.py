@@ -216,7 +216,7 @@ def _fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
         o = native.lanczos_matvec(n, params, op.sigma, wbuf, w_scale,
                                   basis[j - 1] if j > 0 else None,
                                   scales[j - 1] if j > 0 else 0.0, a, basis[j], scales[j], b, r,
-                                  complex_coef=ham.complex, **kw).cpu().numpy()
+                                  complex_coef=ham.complex, keep_diag=True, **kw).cpu().numpy()
         KrylovStats.matvecs += 1
         n2 = math.sqrt(max(o[0], 0.0))
         T[j + 1, j] = n2
@@ -326,8 +326,10 @@ def _pipelined_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: fl
         r = new_vec(j + 2)
         outs.append(r)
         w = wbuf_of(j)
+        # the first-pass diagonal table of the opening hpsi call serves every iteration
         native.lanczos_matvec_dev(n, params, sig, w, basis_at(j - 1) if j > 0 else None, basis_at(j),
-                                  dev[j, 16:22], r, dev[j, 0:12], complex_coef=ham.complex, **kw)
+                                  dev[j, 16:22], r, dev[j, 0:12], complex_coef=ham.complex,
+                                  keep_diag=True, **kw)
         native.lanczos_next_coef(dev[j, 0:12], sig, dev[j, 16:22], dev[j + 1, 16:22], dev[j, 24:30])
         host[j].copy_(dev[j], non_blocking=True)
         ev = torch.cuda.Event()

@@ -41,12 +41,21 @@ extern "C" {
 #define EMU_SV_C128 0
 #define EMU_SV_C64 1
 
-/* emu_sv_hpsi flags: bit 0 complex coefficients, bit 1 noise, bits 8-11
- * kernel kind */
+/* emu_sv_hpsi flags: bit 0 complex coefficients, bit 1 noise, bit 2 keep
+ * the diagonal table, bits 8-11 kernel kind */
 #define EMU_SV_FLAG_COMPLEX 1 /* some phi != 0: complex flip coefficients */
 #define EMU_SV_FLAG_NOISE 2   /* the block carries the non-Hermitian term   */
                               /* -i/2 sum L^+L (emu_sv_pack_noise); implies */
                               /* complex coefficients and the LDS kernel    */
+/* The first pass's diagonal constants are tabulated in the workspace by a
+ * small kernel before the passes.  With EMU_SV_FLAG_KEEP_DIAG (emu_sv_hpsi,
+ * emu_sv_lanczos_matvec, emu_sv_lanczos_matvec_dev) the table is reused when
+ * the workspace's last table was built for the same parameter-block pointer,
+ * n and first-pass tile size; otherwise it is rebuilt.  The caller asserts
+ * that the block's contents have not changed since, and that no call with
+ * another n has used the workspace in between.  The Krylov loop of one time
+ * step sets it on every iteration after the first. */
+#define EMU_SV_FLAG_KEEP_DIAG 4
 #define EMU_SV_KERNEL_DEFAULT (0 << 8) /* library choice (the wave kernel)  */
 #define EMU_SV_KERNEL_LDS (1 << 8)     /* workgroup tiles staged in LDS     */
 #define EMU_SV_KERNEL_WAVE (2 << 8)    /* wavefront tiles in registers,     */
@@ -121,7 +130,8 @@ size_t emu_sv_workspace_bytes(int n, int dtype);
  *            (v and prev taken with their scales); NULL = no reductions.
  *   workspace device scratch of emu_sv_workspace_bytes(n, dtype) bytes,
  *            zero-filled before its first use (it holds a counter that the
- *            library re-arms after every reduction).
+ *            library re-arms after every reduction, and the first pass's
+ *            diagonal table); required, also without reductions.
  *   tile_log / min_contig: log2 of the tile (the amplitudes one workgroup
  *            (LDS kernel) or one wavefront (wave kernel) owns) and the
  *            minimum contiguous low bits of the high-bit passes; 0 = default.

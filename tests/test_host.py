@@ -59,6 +59,14 @@ def test_error_codes_map_to_exceptions():
                                 0, 0, None)
     assert rc == native.EMU_SV_EINVAL
     assert "null" in native.LIB.emu_sv_last_error().decode()
+    # the workspace is required even without reductions (the first pass's
+    # diagonal table lives there); fake device pointers are never touched
+    import ctypes
+    fake, fake2 = ctypes.c_void_p(4096), ctypes.c_void_p(8192)
+    rc = native.LIB.emu_sv_hpsi(10, 0, fake, 1.0, 0.0, fake, 1.0, None, 1.0, fake2, None, None, 0,
+                                0, 0, None)
+    assert rc == native.EMU_SV_EINVAL
+    assert "workspace" in native.LIB.emu_sv_last_error().decode()
 
 
 @pytest.mark.parametrize("m,scale", [(1, 1.0), (3, 0.01), (7, 1.0), (12, 8.0), (40, 30.0)])
