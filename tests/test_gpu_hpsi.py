@@ -117,6 +117,8 @@ TILE_SHAPES = [
     (23, 11, 4, 0, 0, 3),       # A + B + B, no partners
     (22, 13, 4, 4, 11, 2),      # 2^13 B tiles (512 threads) after a 2^11 first pass
     (23, 12, 5, 3, 12, 3),      # A + B + B (3 leftover bits exceed no partner count <= 3)
+    (22, 13, 2, 2, 10, 2),      # 2^13 B tiles with 3 contiguous bits + 2 partner bits
+    (23, 12, 4, 6, 10, 2),      # 2^12 B tiles + 6 partner bits
 ]
 
 
