@@ -60,6 +60,8 @@ SIGNATURES = {
                                    _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "emu_sv_lanczos_matvec_dev": (_i, [_i, _i, _vp, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                        _i, _i, _i, _vp]),
+    "emu_sv_lanczos_iterate_dev": (_i, [_i, _i, _vp, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                        _vp, _vp, _i, _i, _i, _vp]),
     "emu_sv_lanczos_next_coef": (_i, [_vp, _vp, _d, _d, _vp, _vp, _vp, _vp]),
     "emu_sv_lincomb": (_i, [_i, _i, _i, ctypes.POINTER(_vp), _dp, _vp, _vp]),
     "emu_sv_dot": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp]),
@@ -299,6 +301,26 @@ def lanczos_matvec_dev(n: int, ham_dev: torch.Tensor, sigma: complex, w: torch.T
                                         _ptr(r), _ptr(rec[0:5]), _ptr(rec[5:12]), _ptr(
                                             Workspace.get(w.device, n).scratch), tile_log,
                                         min_contig, flags, _stream(w.device)), "lanczos_matvec_dev")
+
+
+def lanczos_iterate_dev(n: int, ham_dev: torch.Tensor, sigma: complex, w: torch.Tensor,
+                        pa: torch.Tensor | None, pb: torch.Tensor, coef: torch.Tensor,
+                        r: torch.Tensor, rec: torch.Tensor, nxt: torch.Tensor, tout: torch.Tensor,
+                        complex_coef: bool = True, tile_log: int = 0, min_contig: int = 0,
+                        partners: int = 0, first_log: int = 0, kernel: str = "tile",
+                        keep_diag: bool = False) -> None:
+    """lanczos_matvec_dev followed by lanczos_next_coef in the same launch
+    sequence: the last reduction also writes the next iteration's
+    coefficients (nxt, 6 doubles) and tout (n2, b, a, ww)."""
+    flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags(kernel, partners, first_log)
+    if keep_diag:
+        flags |= EMU_SV_FLAG_KEEP_DIAG
+    check(LIB.emu_sv_lanczos_iterate_dev(n, dtype_code(w.dtype), _ptr(ham_dev), float(sigma.real),
+                                         float(sigma.imag), _ptr(w), _ptr(pa), _ptr(pb), _ptr(coef),
+                                         _ptr(r), _ptr(rec[0:5]), _ptr(rec[5:12]), _ptr(nxt),
+                                         _ptr(tout), _ptr(Workspace.get(w.device, n).scratch),
+                                         tile_log, min_contig, flags, _stream(w.device)),
+          "lanczos_iterate_dev")
 
 
 def lanczos_next_coef(rec: torch.Tensor, sigma: complex, prev: torch.Tensor, nxt: torch.Tensor,

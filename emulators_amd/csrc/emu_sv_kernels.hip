@@ -1738,6 +1738,16 @@ int launch_pass(const PassArgs &a, hipStream_t st) {
 double *partials2_ptr(void *ws, int n);
 double *partials_ptr(void *ws, int n);
 int finalize(const double *rows, int nrow, int K, void *ws, int n, double *out, hipStream_t st);
+// Outputs of the next-iteration coefficients formed by the last reduction
+// of a pipelined Lanczos iteration (next == nullptr: none).
+struct NextCoefOut {
+  const double *prev = nullptr;  // the coefficient record this iteration read
+  double *next = nullptr, *tout = nullptr;
+  double sig_re = 0.0, sig_im = 0.0;
+};
+int finalize_pair(const double *rows_a, int nrow_a, int ka, double *out_a, const double *rows_b,
+                  int nrow_b, int kb, double *out_b, void *ws, int n, const NextCoefOut &nc,
+                  hipStream_t st);
 const double *dtab_ptr(void *ws, int n);
 
 template <typename T, int KREG, bool CPLX, bool DOTS>
@@ -1761,6 +1771,7 @@ struct OrthoIn {
   double ca_re, ca_im, cb_re, cb_im, pb_scale;
   double *norm2_out;  // kOrthoSums doubles
   const double *dcoef = nullptr;  // device coefficients (LDS and tile kernels)
+  NextCoefOut nc;                 // next coefficients formed on the device (optional)
 };
 
 // The diagonal table of the first pass lives in the workspace.  Each
@@ -1808,6 +1819,7 @@ int run_hpsi(int n, const double *ham, double sr, double si, const void *v, doub
   const int np = make_plan(n, tl, cmin, plan);
   // pass 1 is the contiguous tile [0, t1)
   if (int rc = prepare_diag(n, plan[0].c + plan[0].m, ham, ws, keep_diag, st)) return rc;
+  int nrow_a = 0;
   for (int p = 0; p < np; ++p) {
     PassArgs a{};
     a.v = v;
@@ -1859,12 +1871,20 @@ int run_hpsi(int n, const double *ham, double sr, double si, const void *v, doub
     if (rc) return rc;
     const int t = a.c + a.m;
     const int nrow = (int)(1ull << (n - t));
+    // a Lanczos iteration reduces its first-pass sums together with the last
+    // pass's dots (one launch); the first-pass rows stay in place until then
     if (a.ortho) {
-      rc = finalize(a.partials2, nrow, kOrthoSums, ws, n, ortho->norm2_out, st);
-      if (rc) return rc;
+      if (dots) {
+        nrow_a = nrow;
+      } else {
+        rc = finalize(a.partials2, nrow, kOrthoSums, ws, n, ortho->norm2_out, st);
+        if (rc) return rc;
+      }
     }
     if (fuse) {
-      rc = finalize(a.partials, nrow, 7, ws, n, dots, st);
+      rc = ortho ? finalize_pair(a.partials2, nrow_a, kOrthoSums, ortho->norm2_out, a.partials, nrow,
+                                 7, dots, ws, n, ortho->nc, st)
+                 : finalize(a.partials, nrow, 7, ws, n, dots, st);
       if (rc) return rc;
     }
   }
@@ -1977,6 +1997,7 @@ int run_tile(int n, const double *ham, double sr, double si, const void *v, doub
   TilePass plan[EMU_SV_MAX_QUBITS];
   const int np = make_tile_plan(n, tla, tlb, cmin, max_partner, plan);
   if (int rc = prepare_diag(n, tla, ham, ws, keep_diag, st)) return rc;
+  int nrow_a = 0;
   for (int p = 0; p < np; ++p) {
     const bool last = p == np - 1;
     TileArgs a{};
@@ -2013,11 +2034,17 @@ int run_tile(int n, const double *ham, double sr, double si, const void *v, doub
     if (rc) return rc;
     const int nrow = (int)(1ull << (n - plan[p].tl));
     if (a.ortho) {
-      rc = finalize(a.partials2, nrow, kOrthoSums, ws, n, ortho->norm2_out, st);
-      if (rc) return rc;
+      if (dots) {
+        nrow_a = nrow;  // reduced with the last pass's dots
+      } else {
+        rc = finalize(a.partials2, nrow, kOrthoSums, ws, n, ortho->norm2_out, st);
+        if (rc) return rc;
+      }
     }
     if (a.dots) {
-      rc = finalize(a.partials, nrow, 7, ws, n, dots, st);
+      rc = ortho ? finalize_pair(a.partials2, nrow_a, kOrthoSums, ortho->norm2_out, a.partials, nrow,
+                                 7, dots, ws, n, ortho->nc, st)
+                 : finalize(a.partials, nrow, 7, ws, n, dots, st);
       if (rc) return rc;
     }
   }
@@ -2173,6 +2200,83 @@ const double *dtab_ptr(void *ws, int n) {
   return reinterpret_cast<const double *>(reinterpret_cast<char *>(ws) + ws_layout(n).dtab);
 }
 
+// The reductions that end a fused Lanczos iteration, in one launch: set A
+// (the orthogonalisation sums of the first pass, KA values per row) and set
+// B (the dots of the last pass), each summed in a fixed order as by
+// finalize2_kernel (blocks [0, nbA) take A, the rest B; the last block adds
+// the block sums).  With next != nullptr the last block also forms the next
+// iteration's device coefficients (lanczos_next_coef_body).
+__global__ __launch_bounds__(kThreads) void finalize_pair_kernel(
+    const double *rows_a, int nrow_a, int ka, double *out_a, int nb_a, const double *rows_b,
+    int nrow_b, int kb, double *out_b, double *stage, unsigned *ticket, NextCoefOut nc) {
+  __shared__ double red[kThreads];
+  __shared__ double res[2][kNumDots];
+  __shared__ bool last;
+  const bool is_a = (int)blockIdx.x < nb_a;
+  const int blk = is_a ? (int)blockIdx.x : (int)blockIdx.x - nb_a;
+  const int nblk = is_a ? nb_a : (int)gridDim.x - nb_a;
+  const double *rows = is_a ? rows_a : rows_b;
+  const int nrow = is_a ? nrow_a : nrow_b;
+  const int per = (nrow + nblk - 1) / nblk;
+  const int r0 = blk * per;
+  const int r1 = min(nrow, r0 + per);
+  const int v = threadIdx.x & 7, rl = threadIdx.x >> 3;  // 32 row lanes x 8 values
+  double acc = 0.0;
+  for (int r = r0 + rl; r < r1; r += kThreads / 8) acc += rows[(uint64_t)r * kNumDots + v];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int hlf = kThreads / 2; hlf >= 8; hlf >>= 1) {
+    if (threadIdx.x < hlf) red[threadIdx.x] += red[threadIdx.x + hlf];
+    __syncthreads();
+  }
+  if (threadIdx.x < kNumDots) stage[blockIdx.x * kNumDots + threadIdx.x] = red[threadIdx.x];
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  for (int set = 0; set < 2; ++set) {
+    const unsigned b0 = set == 0 ? 0u : (unsigned)nb_a;
+    const unsigned b1 = set == 0 ? (unsigned)nb_a : gridDim.x;
+    double sm = 0.0;
+    for (unsigned b = b0 + rl; b < b1; b += kThreads / 8)
+      sm += __hip_atomic_load(&stage[b * kNumDots + v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    red[threadIdx.x] = sm;
+    __syncthreads();
+    for (int hlf = kThreads / 2; hlf >= 8; hlf >>= 1) {
+      if (threadIdx.x < hlf) red[threadIdx.x] += red[threadIdx.x + hlf];
+      __syncthreads();
+    }
+    if (threadIdx.x < kNumDots) res[set][threadIdx.x] = red[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x < ka) out_a[threadIdx.x] = res[0][threadIdx.x];
+  if (threadIdx.x < kb) out_b[threadIdx.x] = res[1][threadIdx.x];
+  if (threadIdx.x == 0) {
+    if (nc.next) lanczos_next_coef_body(res[0], res[1], nc.sig_re, nc.sig_im, nc.prev, nc.next, nc.tout);
+    atomicExch(ticket, 0u);
+  }
+}
+
+int finalize_pair(const double *rows_a, int nrow_a, int ka, double *out_a, const double *rows_b,
+                  int nrow_b, int kb, double *out_b, void *ws, int n, const NextCoefOut &nc,
+                  hipStream_t st) {
+  const WsLayout l = ws_layout(n);
+  char *base = reinterpret_cast<char *>(ws);
+  auto blocks = [](int nrow) {
+    int nb = (nrow + 31) / 32;
+    if (nb > kFinalizeBlocks / 2) nb = kFinalizeBlocks / 2;
+    return nb < 1 ? 1 : nb;
+  };
+  const int nb_a = blocks(nrow_a), nb_b = blocks(nrow_b);
+  hipLaunchKernelGGL(finalize_pair_kernel, dim3(nb_a + nb_b), dim3(kThreads), 0, st, rows_a, nrow_a,
+                     ka, out_a, nb_a, rows_b, nrow_b, kb, out_b,
+                     reinterpret_cast<double *>(base + l.stage),
+                     reinterpret_cast<unsigned *>(base + l.ticket), nc);
+  return hip_check(hipGetLastError(), "finalize launch");
+}
+
 int finalize(const double *rows, int nrow, int K, void *ws, int n, double *out, hipStream_t st) {
   const WsLayout l = ws_layout(n);
   char *base = reinterpret_cast<char *>(ws);
@@ -2225,6 +2329,57 @@ int check_tile_schedule(int n, int dtype, int &tl, int &cmin, int &tla, const ch
   if (cmin < 1 || cmin >= tl)
     return fail(EMU_SV_EINVAL, std::string(what) + ": min_contig must be in [1, tile_log)");
   return (n < tl || n < tla) ? 1 : 0;
+}
+
+int lanczos_matvec_dev_impl(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
+                            void *w, const void *pa, const void *pb, const double *coef, void *r,
+                            double *norm2_out, double *dots_out, void *workspace, int tile_log,
+                            int min_contig, int flags, hipStream_t stream, const NextCoefOut &nc) {
+  if (int rc = check_n(n)) return rc;
+  if (int rc = check_dtype(dtype)) return rc;
+  if (!ham || !w || !r || !coef || !norm2_out || !dots_out || !workspace)
+    return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: null pointer");
+  if (w == r || w == pa || w == pb || r == pa || r == pb)
+    return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: w, r, pa and pb must be distinct buffers");
+  const int kind = kernel_kind(flags);
+  if (kind != kKernelTile && kind != kKernelLds)
+    return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: needs the tile or LDS kernel kind");
+  if (flags & EMU_SV_FLAG_NOISE)
+    return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: the Lanczos recurrence needs a Hermitian H (no noise term)");
+  int tla = 0;
+  if (kind == kKernelTile) {
+    tla = tile_first_log(flags, 0);
+    const int rc = check_tile_schedule(n, dtype, tile_log, min_contig, tla, "lanczos_matvec_dev");
+    if (rc < 0) return rc;
+    if (rc > 0) return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: fewer qubits than one tile");
+  } else {
+    default_tiles(dtype, kind, tile_log, min_contig);
+    if (tile_log < kThreadBits || tile_log > kMaxTileLog || (dtype == EMU_SV_C128 && tile_log > 13) ||
+        min_contig < 0 || min_contig >= tile_log)
+      return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: bad tile_log / min_contig");
+  }
+  OrthoIn o;
+  o.pa = pa;
+  o.pb = pb;
+  o.ca_re = o.ca_im = o.cb_re = o.cb_im = 0.0;
+  o.pb_scale = 0.0;
+  o.norm2_out = norm2_out;
+  o.dcoef = coef;
+  o.nc = nc;
+  const bool cplx = (flags & EMU_SV_FLAG_COMPLEX) != 0;
+  const bool keep = (flags & EMU_SV_FLAG_KEEP_DIAG) != 0;
+  if (kind == kKernelLds) {
+    if (dtype == EMU_SV_C128)
+      return run_hpsi<double>(n, ham, sigma_re, sigma_im, w, 1.0, nullptr, 0.0, r, dots_out, workspace,
+                              kind, tile_log, min_contig, cplx, stream, &o, false, keep);
+    return run_hpsi<float>(n, ham, sigma_re, sigma_im, w, 1.0, nullptr, 0.0, r, dots_out, workspace,
+                           kind, tile_log, min_contig, cplx, stream, &o, false, keep);
+  }
+  if (dtype == EMU_SV_C128)
+    return run_tile<double>(n, ham, sigma_re, sigma_im, w, 1.0, nullptr, 0.0, r, dots_out, workspace,
+                            tla, tile_log, min_contig, tile_partners(flags), cplx, stream, &o, keep);
+  return run_tile<float>(n, ham, sigma_re, sigma_im, w, 1.0, nullptr, 0.0, r, dots_out, workspace, tla,
+                         tile_log, min_contig, tile_partners(flags), cplx, stream, &o, keep);
 }
 
 }  // namespace
@@ -2422,50 +2577,26 @@ int emu_sv_lanczos_matvec_dev(int n, int dtype, const double *ham, double sigma_
                               void *w, const void *pa, const void *pb, const double *coef, void *r,
                               double *norm2_out, double *dots_out, void *workspace, int tile_log,
                               int min_contig, int flags, hipStream_t stream) {
-  if (int rc = check_n(n)) return rc;
-  if (int rc = check_dtype(dtype)) return rc;
-  if (!ham || !w || !r || !coef || !norm2_out || !dots_out || !workspace)
-    return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: null pointer");
-  if (w == r || w == pa || w == pb || r == pa || r == pb)
-    return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: w, r, pa and pb must be distinct buffers");
-  const int kind = kernel_kind(flags);
-  if (kind != kKernelTile && kind != kKernelLds)
-    return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: needs the tile or LDS kernel kind");
-  if (flags & EMU_SV_FLAG_NOISE)
-    return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: the Lanczos recurrence needs a Hermitian H (no noise term)");
-  int tla = 0;
-  if (kind == kKernelTile) {
-    tla = tile_first_log(flags, 0);
-    const int rc = check_tile_schedule(n, dtype, tile_log, min_contig, tla, "lanczos_matvec_dev");
-    if (rc < 0) return rc;
-    if (rc > 0) return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: fewer qubits than one tile");
-  } else {
-    default_tiles(dtype, kind, tile_log, min_contig);
-    if (tile_log < kThreadBits || tile_log > kMaxTileLog || (dtype == EMU_SV_C128 && tile_log > 13) ||
-        min_contig < 0 || min_contig >= tile_log)
-      return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: bad tile_log / min_contig");
-  }
-  OrthoIn o;
-  o.pa = pa;
-  o.pb = pb;
-  o.ca_re = o.ca_im = o.cb_re = o.cb_im = 0.0;
-  o.pb_scale = 0.0;
-  o.norm2_out = norm2_out;
-  o.dcoef = coef;
-  const bool cplx = (flags & EMU_SV_FLAG_COMPLEX) != 0;
-  const bool keep = (flags & EMU_SV_FLAG_KEEP_DIAG) != 0;
-  if (kind == kKernelLds) {
-    if (dtype == EMU_SV_C128)
-      return run_hpsi<double>(n, ham, sigma_re, sigma_im, w, 1.0, nullptr, 0.0, r, dots_out, workspace,
-                              kind, tile_log, min_contig, cplx, stream, &o, false, keep);
-    return run_hpsi<float>(n, ham, sigma_re, sigma_im, w, 1.0, nullptr, 0.0, r, dots_out, workspace,
-                           kind, tile_log, min_contig, cplx, stream, &o, false, keep);
-  }
-  if (dtype == EMU_SV_C128)
-    return run_tile<double>(n, ham, sigma_re, sigma_im, w, 1.0, nullptr, 0.0, r, dots_out, workspace,
-                            tla, tile_log, min_contig, tile_partners(flags), cplx, stream, &o, keep);
-  return run_tile<float>(n, ham, sigma_re, sigma_im, w, 1.0, nullptr, 0.0, r, dots_out, workspace, tla,
-                         tile_log, min_contig, tile_partners(flags), cplx, stream, &o, keep);
+  return lanczos_matvec_dev_impl(n, dtype, ham, sigma_re, sigma_im, w, pa, pb, coef, r, norm2_out,
+                                 dots_out, workspace, tile_log, min_contig, flags, stream,
+                                 NextCoefOut{});
+}
+
+int emu_sv_lanczos_iterate_dev(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
+                               void *w, const void *pa, const void *pb, const double *coef, void *r,
+                               double *norm2_out, double *dots_out, double *next_coef,
+                               double *t_out, void *workspace, int tile_log, int min_contig,
+                               int flags, hipStream_t stream) {
+  if (!next_coef || !t_out) return fail(EMU_SV_EINVAL, "lanczos_iterate_dev: null pointer");
+  if (next_coef == coef) return fail(EMU_SV_EINVAL, "lanczos_iterate_dev: next_coef must not alias coef");
+  NextCoefOut nc;
+  nc.prev = coef;
+  nc.next = next_coef;
+  nc.tout = t_out;
+  nc.sig_re = sigma_re;
+  nc.sig_im = sigma_im;
+  return lanczos_matvec_dev_impl(n, dtype, ham, sigma_re, sigma_im, w, pa, pb, coef, r, norm2_out,
+                                 dots_out, workspace, tile_log, min_contig, flags, stream, nc);
 }
 
 int emu_sv_lanczos_next_coef(const double *norm2, const double *dots, double sigma_re, double sigma_im,

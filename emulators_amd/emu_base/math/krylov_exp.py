@@ -326,11 +326,11 @@ def _pipelined_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: fl
         r = new_vec(j + 2)
         outs.append(r)
         w = wbuf_of(j)
-        # the first-pass diagonal table of the opening hpsi call serves every iteration
-        native.lanczos_matvec_dev(n, params, sig, w, basis_at(j - 1) if j > 0 else None, basis_at(j),
-                                  dev[j, 16:22], r, dev[j, 0:12], complex_coef=ham.complex,
-                                  keep_diag=True, **kw)
-        native.lanczos_next_coef(dev[j, 0:12], sig, dev[j, 16:22], dev[j + 1, 16:22], dev[j, 24:30])
+        # the first-pass diagonal table of the opening hpsi call serves every
+        # iteration; the last reduction also forms iteration j+1's coefficients
+        native.lanczos_iterate_dev(n, params, sig, w, basis_at(j - 1) if j > 0 else None, basis_at(j),
+                                   dev[j, 16:22], r, dev[j, 0:12], dev[j + 1, 16:22], dev[j, 24:30],
+                                   complex_coef=ham.complex, keep_diag=True, **kw)
         host[j].copy_(dev[j], non_blocking=True)
         ev = torch.cuda.Event()
         ev.record(stream)

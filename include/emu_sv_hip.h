@@ -177,6 +177,18 @@ int emu_sv_lanczos_matvec_dev(int n, int dtype, const double *ham, double sigma_
                               double *dots_out, void *workspace, int tile_log,
                               int min_contig, int flags, hipStream_t stream);
 
+/* emu_sv_lanczos_matvec_dev and emu_sv_lanczos_next_coef in one launch
+ * sequence: the reduction that ends the iteration also writes next_coef (6
+ * doubles, device; must not alias coef) and t_out (6 doubles) exactly as
+ * emu_sv_lanczos_next_coef(norm2_out, dots_out, sigma, coef, next_coef,
+ * t_out) would.  Replaces the same reference code as the two calls. */
+int emu_sv_lanczos_iterate_dev(int n, int dtype, const double *ham, double sigma_re,
+                               double sigma_im, void *w, const void *pa, const void *pb,
+                               const double *coef, void *r, double *norm2_out,
+                               double *dots_out, double *next_coef, double *t_out,
+                               void *workspace, int tile_log, int min_contig, int flags,
+                               hipStream_t stream);
+
 /* On the device (one thread): the coefficients of the next fused Lanczos
  * iteration from the reductions of the current one (norm2 = the 5
  * orthogonalisation sums, dots = ||r||^2, <z|r>) and prev (the current

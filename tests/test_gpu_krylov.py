@@ -308,3 +308,46 @@ def test_pipelined_engine_matches_unpipelined(gpu_device, monkeypatch, dtype, n,
     out = kx.krylov_exp(op, v.clone(), tol, tol)
     assert calls, "krylov_exp did not take the pipelined engine"
     assert float((out.cpu().to(torch.complex128) - a).norm()) < tolv
+
+
+@pytest.mark.parametrize("kernel,sched", [("tile", dict(tile_log=11, min_contig=4, first_log=11)),
+                                          ("lds", dict(tile_log=11, min_contig=2))])
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_iterate_dev_equals_matvec_dev_then_next_coef(gpu_device, kernel, sched, dtype):
+    """emu_sv_lanczos_iterate_dev (next coefficients formed by the iteration's
+    last reduction) against emu_sv_lanczos_matvec_dev + emu_sv_lanczos_next_coef
+    on the same inputs: identical r, z, reductions, next coefficients and
+    side outputs, bit for bit."""
+    from emulators_amd import _native as native
+    from emulators_amd.emu_sv import RydbergHamiltonian
+    n = 15
+    g = torch.Generator().manual_seed(91)
+    om = torch.rand(n, generator=g, dtype=torch.float64) * 10
+    de = (torch.rand(n, generator=g, dtype=torch.float64) - 0.5) * 10
+    ph = torch.rand(n, generator=g, dtype=torch.float64)
+    u = torch.rand(n, n, generator=g, dtype=torch.float64) * 20
+    ham = RydbergHamiltonian(om, de, ph, u + u.T, device=gpu_device)
+    params = ham.device_params(gpu_device)
+    sig = -0.01j
+    vecs = [torch.randn(2**n, dtype=torch.complex128, generator=g).to(gpu_device, dtype)
+            for _ in range(3)]
+    coef = torch.tensor([0.01, -0.02, 0.3, 0.05, 0.7, 1.3], dtype=torch.float64, device=gpu_device)
+    outs = []
+    for fused in (False, True):
+        w, pa, pb = (x.clone() for x in vecs)
+        r = torch.empty_like(w)
+        rec = torch.zeros(12, dtype=torch.float64, device=gpu_device)
+        nxt = torch.zeros(6, dtype=torch.float64, device=gpu_device)
+        tout = torch.zeros(6, dtype=torch.float64, device=gpu_device)
+        if fused:
+            native.lanczos_iterate_dev(n, params, sig, w, pa, pb, coef, r, rec, nxt, tout,
+                                       complex_coef=ham.complex, kernel=kernel, **sched)
+        else:
+            native.lanczos_matvec_dev(n, params, sig, w, pa, pb, coef, r, rec,
+                                      complex_coef=ham.complex, kernel=kernel, **sched)
+            native.lanczos_next_coef(rec, sig, coef, nxt, tout)
+        torch.cuda.synchronize()
+        outs.append((w, r, rec, nxt, tout))
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
+    assert float(outs[1][2][0]) > 0.0 and torch.all(torch.isfinite(outs[1][3]))
