@@ -256,7 +256,9 @@ bool tridiag_eig(int n, double *d, double *e, int nrow, const int *rows, double 
     const double mu = d[hi] - eh * eh / (dd + std::copysign(std::hypot(dd, eh), dd));
     double x = d[lo] - mu, z = e[lo], bulge = 0.0;
     for (int k = lo; k < hi; ++k) {
-      const double r = std::hypot(x, z);
+      // |x|, |z| are bounded by the matrix entries (|J| << 1e150), so the
+      // plain root cannot overflow; it is several times cheaper than hypot
+      const double r = std::sqrt(x * x + z * z);
       const double c = r == 0.0 ? 1.0 : x / r, s = r == 0.0 ? 0.0 : z / r;
       if (k > lo) e[k - 1] = r;
       const double a = d[k], b = e[k], d1 = d[k + 1];
@@ -280,21 +282,36 @@ bool tridiag_eig(int n, double *d, double *e, int nrow, const int *rows, double 
   return true;
 }
 
-// phi_p(z) for p = 1, 2: series near 0, closed form elsewhere.
-cd phi(int p, cd z) {
-  if (std::abs(z) < 0.5) {
+// phi_p(z) for p = 1, 2: series near 0, closed form elsewhere.  Written
+// in real arithmetic: std::complex products and quotients go through the
+// library's NaN-aware helpers, which dominated the cost of this routine.
+cd phi(int p, cd zc) {
+  const double zr = zc.real(), zi = zc.imag();
+  if (zr * zr + zi * zi < 0.25) {
     // sum_k z^k / (k + p)!
-    cd term = 1.0, sum = 0.0;
+    double tr = 1.0, ti = 0.0, sr = 0.0, si = 0.0;
     double fact = p == 1 ? 1.0 : 2.0;  // p!
     for (int k = 0; k < 30; ++k) {
-      sum += term / fact;
-      term *= z;
+      sr += tr / fact;
+      si += ti / fact;
+      const double nr = tr * zr - ti * zi;
+      ti = tr * zi + ti * zr;
+      tr = nr;
       fact *= (k + p + 1);
     }
-    return sum;
+    return cd(sr, si);
   }
-  const cd ez = std::exp(z);
-  return p == 1 ? (ez - 1.0) / z : (ez - 1.0 - z) / (z * z);
+  const double ea = std::exp(zr);
+  double nr = ea * std::cos(zi) - 1.0, ni = ea * std::sin(zi);  // e^z - 1
+  double dr = zr, di = zi;                                      // z
+  if (p == 2) {
+    nr -= zr;  // e^z - 1 - z
+    ni -= zi;
+    dr = zr * zr - zi * zi;  // z^2
+    di = 2.0 * zr * zi;
+  }
+  const double den = dr * dr + di * di;
+  return cd((nr * dr + ni * di) / den, (ni * dr - nr * di) / den);
 }
 
 }  // namespace
