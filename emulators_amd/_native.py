@@ -164,6 +164,25 @@ def _require_device_vec(vec: torch.Tensor, what: str) -> None:
         raise ValueError(f"{what}: tensor must be contiguous")
 
 
+def _require_doubles(t: torch.Tensor, count: int, what: str) -> None:
+    """A contiguous float64 device tensor of at least ``count`` elements."""
+    if not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous() or t.numel() < count:
+        raise ValueError(f"{what}: needs a contiguous float64 device tensor of >= {count} elements")
+
+
+def _require_iteration(n: int, ham_dev: torch.Tensor, w: torch.Tensor, pa: torch.Tensor | None,
+                       pb: torch.Tensor, r: torch.Tensor, coef: torch.Tensor, rec: torch.Tensor,
+                       what: str) -> None:
+    """Shapes of one device-coefficient Lanczos iteration, checked before any launch."""
+    _require_doubles(ham_dev, EMU_SV_HAM_DOUBLES, what + " (parameter block)")
+    for t in (w, pb, r) + ((pa,) if pa is not None else ()):
+        _require_device_vec(t, what)
+        if t.dtype != w.dtype or t.numel() != (1 << n):
+            raise ValueError(f"{what}: vectors must be 2**n elements of one dtype")
+    _require_doubles(coef, 6, what + " (coefficients)")
+    _require_doubles(rec, 12, what + " (reductions)")
+
+
 def pack_hamiltonian(omega, delta, phi, interaction) -> tuple[np.ndarray, bool]:
     """Host packing of the parameter block (real float64 per qubit)."""
     om = np.ascontiguousarray(np.asarray(omega, dtype=np.float64))
@@ -243,6 +262,7 @@ def hpsi(n: int, ham_dev: torch.Tensor, sigma: complex, v: torch.Tensor, v_scale
         raise ValueError("hpsi: v and w must be 2**n-element vectors of one dtype")
     if prev is not None and (prev.dtype != v.dtype or prev.numel() != v.numel()):
         raise ValueError("hpsi: prev must match v")
+    _require_doubles(ham_dev, EMU_SV_HAM_DOUBLES, "hpsi (parameter block)")
     ws = Workspace.get(v.device, n)
     out = ws.results[0:7] if dots else None
     flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags(kernel, partners, first_log)
@@ -270,6 +290,7 @@ def lanczos_matvec(n: int, ham_dev: torch.Tensor, sigma: complex, w: torch.Tenso
         _require_device_vec(t, "lanczos_matvec")
         if t.dtype != w.dtype or t.numel() != (1 << n):
             raise ValueError("lanczos_matvec: vectors must be 2**n elements of one dtype")
+    _require_doubles(ham_dev, EMU_SV_HAM_DOUBLES, "lanczos_matvec (parameter block)")
     ws = Workspace.get(w.device, n)
     out = ws.results[24:36]
     flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags(kernel, partners, first_log)
@@ -293,6 +314,7 @@ def lanczos_matvec_dev(n: int, ham_dev: torch.Tensor, sigma: complex, w: torch.T
     b*pb_scale, pb_scale, w_scale) for the tile or LDS kernel; rec (device,
     >= 12 doubles) receives [||z||^2, <z|pb_s>, <z|w_s>, ||r||^2, <z|r>, ...].
     ``keep_diag``: as for hpsi."""
+    _require_iteration(n, ham_dev, w, pa, pb, r, coef, rec, "lanczos_matvec_dev")
     flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags(kernel, partners, first_log)
     if keep_diag:
         flags |= EMU_SV_FLAG_KEEP_DIAG
@@ -312,6 +334,9 @@ def lanczos_iterate_dev(n: int, ham_dev: torch.Tensor, sigma: complex, w: torch.
     """lanczos_matvec_dev followed by lanczos_next_coef in the same launch
     sequence: the last reduction also writes the next iteration's
     coefficients (nxt, 6 doubles) and tout (n2, b, a, ww)."""
+    _require_iteration(n, ham_dev, w, pa, pb, r, coef, rec, "lanczos_iterate_dev")
+    _require_doubles(nxt, 6, "lanczos_iterate_dev (next coefficients)")
+    _require_doubles(tout, 6, "lanczos_iterate_dev (side outputs)")
     flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags(kernel, partners, first_log)
     if keep_diag:
         flags |= EMU_SV_FLAG_KEEP_DIAG

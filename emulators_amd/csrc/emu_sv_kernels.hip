@@ -1793,6 +1793,9 @@ int prepare_diag(int n, int t1, const double *ham, void *ws, bool keep, hipStrea
     if (keep && it != g_diag_tables.end() && it->second.ham == ham && it->second.n == n &&
         it->second.t1 == t1)
       return EMU_SV_OK;
+    // records of workspaces that were freed are never looked up again; a
+    // bounded map (clearing only forces rebuilds) keeps them from piling up
+    if (it == g_diag_tables.end() && g_diag_tables.size() >= 256) g_diag_tables.clear();
     g_diag_tables[ws] = DiagRecord{ham, n, t1};
   }
   const uint64_t ntiles = 1ull << (n - t1);

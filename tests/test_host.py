@@ -442,6 +442,22 @@ def test_new_entry_points_validate_arguments_on_the_host():
                                          None) == native.EMU_SV_EINVAL  # c128 LDS tiles <= 2^13
 
 
+def test_device_iteration_wrappers_check_shapes_before_launch():
+    """The device-coefficient Lanczos wrappers refuse host tensors and
+    undersized records before anything reaches the library."""
+    from emulators_amd import _native as native
+    n = 4
+    ham = torch.zeros(native.EMU_SV_HAM_DOUBLES, dtype=torch.float64)
+    v = torch.zeros(2**n, dtype=torch.complex128)
+    rec = torch.zeros(12, dtype=torch.float64)
+    coef = torch.zeros(6, dtype=torch.float64)
+    with pytest.raises(ValueError, match="parameter block"):
+        native.lanczos_iterate_dev(n, ham, -0.01j, v, None, v.clone(), coef, v.clone(), rec,
+                                   coef.clone(), coef.clone())
+    with pytest.raises(ValueError, match="parameter block"):
+        native.lanczos_matvec_dev(n, ham, -0.01j, v, None, v.clone(), coef, v.clone(), rec)
+
+
 @pytest.mark.parametrize("m", [1, 3, 12, 30, 62])
 @pytest.mark.parametrize("scale", [0.01, 1.0, 8.0, 40.0, 400.0])
 def test_expm_column_vs_scipy(m, scale):
