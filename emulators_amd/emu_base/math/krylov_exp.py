@@ -420,9 +420,14 @@ def _fused_arnoldi(op, v: torch.Tensor, exp_tolerance: float,
         params = ham.device_params(v.device)
         sched = ham._schedule(v)
 
+        built = []
+
         def matvec(src, scale, dst):
+            # the first call of this run builds the diagonal table; the later
+            # ones keep it (the library rebuilds it if another block intervened)
             native.hpsi(n, params, op.sigma, src, scale, dst, complex_coef=ham.complex,
-                        noise=ham.noisy, **sched.kwargs())
+                        noise=ham.noisy, keep_diag=bool(built), **sched.kwargs())
+            built.append(True)
     else:
         matvec = op.device_matvec
     initial_norm = math.sqrt(float(native.dot(n, v)[0].item()))

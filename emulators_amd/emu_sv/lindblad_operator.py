@@ -62,6 +62,7 @@ class RydbergLindbladian:
                                                  torch.cat([ph, z]).numpy(), u2.numpy())
         self._s_full_cache = None  # built on first use, as the reference reads the operators
         self._dev: dict = {}
+        self._diag_built: set = set()  # device blocks whose diagonal table was built once
 
     @property
     def _s_full(self) -> torch.Tensor:
@@ -93,8 +94,13 @@ class RydbergLindbladian:
                       s_op: torch.Tensor | None) -> None:
         """dst = H (scale src) + sum_q S_q (scale src) (flat 2N-bit vectors)."""
         n = self.nqubits
-        native.hpsi(2 * n, self._params(src.device), 1.0, src, scale, dst,
-                    complex_coef=self.complex, **_SCHED.kwargs())
+        params = self._params(src.device)
+        # this block's diagonal table, once built, is reused (the library
+        # rebuilds it whenever another block used the workspace in between)
+        key = params.data_ptr()
+        native.hpsi(2 * n, params, 1.0, src, scale, dst, complex_coef=self.complex,
+                    keep_diag=key in self._diag_built, **_SCHED.kwargs())
+        self._diag_built.add(key)
         if s_op is None:
             return
         op = np.ascontiguousarray((s_op * scale).numpy())
