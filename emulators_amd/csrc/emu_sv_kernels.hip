@@ -964,11 +964,12 @@ __global__ __launch_bounds__(kThreads) void arnoldi_mgs_kernel(uint64_t npk, voi
 }
 
 // The whole modified Gram-Schmidt of a small vector (D <= 2^13) in one
-// workgroup: w stays in registers (kMgsThreads threads, D / kMgsThreads
-// elements each), every step reduces (||w||^2, <u_k|w>) over the workgroup,
-// subtracts s_k^2 <u_k|w> u_k and rounds w to the storage type, as the
-// launch chain does.  One launch instead of 2 (count + 1).
-constexpr int kMgsThreads = 1024;
+// workgroup: w stays in registers (NT threads, EPT = D / NT elements each),
+// every step reduces (||w||^2, <u_k|w>) over the workgroup, subtracts
+// s_k^2 <u_k|w> u_k and rounds w to the storage type, as the launch chain
+// does.  One launch instead of 2 (count + 1).  The steps are sequential, so
+// the cost is the per-step reduction: four waves (eight at D = 2^13) beat
+// sixteen (n = 10: 15 us against 33 us; n = 12: 27 us against 41 us).
 constexpr int kMgsSmallLog = 13;
 constexpr int kMgsSmallCount = 128;
 struct MgsSmallArgs {
@@ -976,9 +977,9 @@ struct MgsSmallArgs {
   double s2[kMgsSmallCount];
 };
 
-template <int K>
+template <int NT, int K>
 __device__ __forceinline__ void mgs_block_sum(double (&v)[K], double *red) {
-  constexpr int NW = kMgsThreads / 64;
+  constexpr int NW = NT / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -996,17 +997,17 @@ __device__ __forceinline__ void mgs_block_sum(double (&v)[K], double *red) {
   __syncthreads();
 }
 
-template <typename T, int EPT>
-__global__ __launch_bounds__(kMgsThreads) void arnoldi_mgs_small_kernel(int count, void *wv,
-                                                                        const MgsSmallArgs args,
-                                                                        double *records) {
+template <typename T, int EPT, int NT>
+__global__ __launch_bounds__(NT) void arnoldi_mgs_small_kernel(int count, void *wv,
+                                                               const MgsSmallArgs args,
+                                                               double *records) {
   using T2 = typename Vec2<T>::type;
-  __shared__ double red[(kMgsThreads / 64) * 3];
+  __shared__ double red[(NT / 64) * 3];
   T2 *w = reinterpret_cast<T2 *>(wv);
   double wr[EPT], wi[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
-    const T2 v = w[threadIdx.x + e * kMgsThreads];
+    const T2 v = w[threadIdx.x + e * NT];
     wr[e] = v.x;
     wi[e] = v.y;
   }
@@ -1015,7 +1016,7 @@ __global__ __launch_bounds__(kMgsThreads) void arnoldi_mgs_small_kernel(int coun
     double ur[EPT], ui[EPT];
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      const T2 q = u[threadIdx.x + e * kMgsThreads];
+      const T2 q = u[threadIdx.x + e * NT];
       ur[e] = q.x;
       ui[e] = q.y;
     }
@@ -1026,7 +1027,7 @@ __global__ __launch_bounds__(kMgsThreads) void arnoldi_mgs_small_kernel(int coun
       acc[1] += ur[e] * wr[e] + ui[e] * wi[e];
       acc[2] += ur[e] * wi[e] - ui[e] * wr[e];
     }
-    mgs_block_sum<3>(acc, red);
+    mgs_block_sum<NT, 3>(acc, red);
     if (threadIdx.x == 0) {
       records[4 * k] = acc[0];
       records[4 * k + 1] = acc[1];
@@ -1044,14 +1045,14 @@ __global__ __launch_bounds__(kMgsThreads) void arnoldi_mgs_small_kernel(int coun
   double nrm[1] = {0.0};
 #pragma unroll
   for (int e = 0; e < EPT; ++e) nrm[0] += wr[e] * wr[e] + wi[e] * wi[e];
-  mgs_block_sum<1>(nrm, red);
+  mgs_block_sum<NT, 1>(nrm, red);
   if (threadIdx.x == 0) records[4 * count] = nrm[0];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     T2 o;
     o.x = static_cast<T>(wr[e]);
     o.y = static_cast<T>(wi[e]);
-    w[threadIdx.x + e * kMgsThreads] = o;
+    w[threadIdx.x + e * NT] = o;
   }
 }
 
@@ -1064,10 +1065,10 @@ int launch_mgs_small(int n, int count, const void *const *vecs, const double *sc
     args.s2[k] = scales[k] * scales[k];
   }
   switch (n) {
-    case 10: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 1>), dim3(1), dim3(kMgsThreads), 0, st, count, w, args, records); break;
-    case 11: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 2>), dim3(1), dim3(kMgsThreads), 0, st, count, w, args, records); break;
-    case 12: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 4>), dim3(1), dim3(kMgsThreads), 0, st, count, w, args, records); break;
-    case 13: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 8>), dim3(1), dim3(kMgsThreads), 0, st, count, w, args, records); break;
+    case 10: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 4, 256>), dim3(1), dim3(256), 0, st, count, w, args, records); break;
+    case 11: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 8, 256>), dim3(1), dim3(256), 0, st, count, w, args, records); break;
+    case 12: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 16, 256>), dim3(1), dim3(256), 0, st, count, w, args, records); break;
+    case 13: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 16, 512>), dim3(1), dim3(512), 0, st, count, w, args, records); break;
     default: return fail(EMU_SV_EINVAL, "arnoldi_orthogonalize: small path needs 10 <= n <= 13");
   }
   return hip_check(hipGetLastError(), "arnoldi_mgs_small launch");

@@ -79,7 +79,7 @@ def test_noisy_expectation_real_part(gpu_device):
     assert abs(float(h.expect(psi)) - float(expected)) < 1e-12
 
 
-@pytest.mark.parametrize("n,count", [(4, 1), (8, 5), (13, 9)])
+@pytest.mark.parametrize("n,count", [(4, 1), (8, 5), (10, 12), (11, 3), (12, 20), (13, 9)])
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
 def test_arnoldi_records_vs_mgs(gpu_device, n, count, dtype):
     from emulators_amd import _native as native
