@@ -145,7 +145,15 @@ def _ptr(t: torch.Tensor | None):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
+# torch's current-stream query through the Stream object costs ~5 us per call;
+# the raw query returns the same handle (at::cuda::getCurrentCUDAStream).
+_RAW_STREAM = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def _stream(device: torch.device):
+    if _RAW_STREAM is not None:
+        idx = device.index if device.index is not None else torch.cuda.current_device()
+        return ctypes.c_void_p(_RAW_STREAM(idx))
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
@@ -406,10 +414,13 @@ def arnoldi_orthogonalize(n: int, basis: list[torch.Tensor], scales: list[float]
     count = len(basis)
     if count != len(scales) or count < 1:
         raise ValueError("arnoldi_orthogonalize: one scale per basis vector")
+    size, dt = 1 << n, w.dtype
+    if w.numel() != size:
+        raise ValueError("arnoldi_orthogonalize: w must have 2**n elements")
     for v in basis:
-        _require_device_vec(v, "arnoldi_orthogonalize")
-        if v.dtype != w.dtype or v.numel() != w.numel() or v.numel() != (1 << n):
-            raise ValueError("arnoldi_orthogonalize: basis vectors must match w")
+        if not (v.is_cuda and v.dtype == dt and v.numel() == size and v.is_contiguous()):
+            raise ValueError("arnoldi_orthogonalize: basis vectors must be contiguous device "
+                             "tensors matching w")
     ws = Workspace.get(w.device, n)
     rec = torch.empty(count + 1, 4, dtype=torch.float64, device=w.device)
     arr = (ctypes.c_void_p * count)(*[v.data_ptr() for v in basis])
