@@ -337,3 +337,24 @@ def test_keep_diag_reuses_only_a_matching_table(gpu_device, kernel, tl, cmin, fi
     ref = OracleHamiltonian(om, de, ph, u) * psi
     tol = 1e-12 if dtype == torch.complex128 else 2e-6
     assert float((fresh1.cpu().to(torch.complex128) - ref).norm() / ref.norm()) < tol
+
+
+def test_stream_handle_is_torchs_current_stream(gpu_device):
+    """The launch stream the bindings pass is torch's current stream, also
+    inside a stream context (work is ordered with the caller's torch ops)."""
+    from emulators_amd import _native as native
+
+    def handle():  # the null (default) stream is 0, which ctypes reports as None
+        return native._stream(gpu_device).value or 0
+
+    assert handle() == torch.cuda.current_stream(gpu_device).cuda_stream
+    side = torch.cuda.Stream(device=gpu_device)
+    with torch.cuda.stream(side):
+        assert handle() == side.cuda_stream != 0
+        # an H.psi enqueued on the side stream matches the default-stream result
+        om, de, ph, u = _params(8, True, 42)
+        psi = torch.randn(2**8, dtype=torch.complex128, generator=torch.Generator().manual_seed(3))
+        got = _hpsi_gpu(8, om, de, ph, u, psi, torch.complex128)
+    side.synchronize()
+    ref = _hpsi_gpu(8, om, de, ph, u, psi, torch.complex128)
+    assert torch.equal(got, ref)
