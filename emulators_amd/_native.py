@@ -243,6 +243,9 @@ class Workspace:
         # zero-filled: holds the reduction ticket counter, which the library re-arms
         self.scratch = torch.zeros(nbytes, dtype=torch.uint8, device=device)
         self.results = torch.zeros(64, dtype=torch.float64, device=device)
+        # the zero fill ran on the current stream; later calls may come from
+        # another stream, so it must be complete before the workspace is shared
+        torch.cuda.current_stream(device).synchronize()
         self.n = n
 
     @classmethod
