@@ -1,0 +1,96 @@
+"""The N=26 triangular-lattice workload at its full size (2^26 amplitudes).
+
+Direct parity of H.psi with the oracle at full size (the oracle's torch-CPU
+H.psi takes about a second there), for the schedules the autotuner picks
+between, and size-independent properties of full Krylov steps: norm
+preservation, energy conservation under the step's constant H, complex64
+against complex128, and the pipelined engine against the one-read-back
+engine."""
+
+import importlib
+import os
+
+import pytest
+import torch
+
+from oracle import emu_sv_oracle as orc
+
+pytestmark = pytest.mark.gpu
+N = 26
+SCHEDULES = {
+    torch.complex128: [("lds", 11, 2, 0, 0), ("tile", 12, 4, 0, 11), ("tile", 13, 6, 0, 12)],
+    torch.complex64: [("lds", 11, 3, 0, 0), ("tile", 13, 6, 0, 12), ("tile", 12, 5, 0, 12)],
+}
+
+
+@pytest.fixture(scope="module")
+def step150():
+    from emulators_amd import workloads
+    seq = workloads.triangular26()
+    k = seq.omega.shape[0] // 2
+    return seq.omega[k], seq.delta[k], seq.phi[k], seq.interaction_matrix(0.0)[0]
+
+
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_hpsi_full_size_matches_oracle(gpu_device, step150, dtype):
+    """H.psi on a seeded random 2^26 vector: every schedule the autotuner
+    picks between against the oracle's index_add_ product, relative error
+    <= 1e-12 (complex128) or 2e-6 (complex64 vectors, FP64 diagonal)."""
+    from emulators_amd import _native as native
+    from emulators_amd.emu_sv import RydbergHamiltonian
+    om, de, ph, u = step150
+    torch.set_num_threads(min(16, os.cpu_count() or 1))
+    psi = torch.randn(2**N, dtype=torch.complex128, generator=torch.Generator().manual_seed(2026))
+    psi = psi.to(dtype).to(torch.complex128)  # the input the kernels see
+    ref = (orc.OracleHamiltonian(om, de, ph, u) * psi).to(gpu_device)
+    ham = RydbergHamiltonian(om, de, ph, u, device=gpu_device)
+    params = ham.device_params(gpu_device)
+    v = psi.to(gpu_device, dtype)
+    del psi
+    w = torch.empty_like(v)
+    rn = ref.norm()
+    tol = 1e-12 if dtype == torch.complex128 else 2e-6
+    for kind, tl, cmin, pr, fl in SCHEDULES[dtype]:
+        native.hpsi(N, params, 1.0, v, 1.0, w, complex_coef=ham.complex, kernel=kind,
+                    tile_log=tl, min_contig=cmin, partners=pr, first_log=fl)
+        err = float((w.to(torch.complex128) - ref).norm() / rn)
+        assert err < tol, (kind, tl, cmin, fl, err)
+
+
+@pytest.mark.parametrize("kernel", ["lds", "tile"])
+def test_krylov_step_full_size_properties(gpu_device, step150, monkeypatch, kernel):
+    """One 10 ns step of the workload from |g...g> and one 1 ns step from a
+    random state, on both schedule kinds: unit norm (1e-9), energy conserved
+    under the step's constant H, the pipelined engine equal to the
+    one-read-back engine, and complex64 within 1-F < 1e-10 of complex128."""
+    from emulators_amd.emu_sv import RydbergHamiltonian
+    kx = importlib.import_module("emulators_amd.emu_base.math.krylov_exp")
+    monkeypatch.setenv("EMU_SV_SCHEDULE", "c128=lds:11:2,c64=lds:11:3" if kernel == "lds"
+                       else "c128=tile:12:4:0:11,c64=tile:13:6:0:12")
+    om, de, ph, u = step150
+    ham = RydbergHamiltonian(om, de, ph, u, device=gpu_device)
+    ground = torch.zeros(2**N, dtype=torch.complex128, device=gpu_device)
+    ground[0] = 1.0
+    rnd = torch.randn(2**N, dtype=torch.complex128, device=gpu_device,
+                      generator=torch.Generator(device=gpu_device).manual_seed(7))
+    rnd /= rnd.norm()
+    for v, dt in ((ground, 0.01), (rnd, 0.001)):
+        op = kx.ScaledHamiltonian(ham, -1j * dt)
+        e0 = float(ham.expect(v))
+        piped = kx._pipelined_lanczos(op, v.clone(), 1e-10, 1e-10, 100,
+                                      kx.autotune.best_schedule(N, v.dtype, gpu_device,
+                                                                ham.device_params(gpu_device),
+                                                                ham.complex))
+        plain = kx._fused_lanczos(op, v.clone(), 1e-10, 1e-10, 100)
+        assert piped.converged and plain.converged
+        assert piped.iteration_count == plain.iteration_count
+        out = piped.combine()
+        assert float((out - plain.combine()).norm()) < 1e-12
+        assert abs(float(out.norm()) - 1.0) < 1e-9
+        e1 = float(ham.expect(out))
+        assert abs(e1 - e0) < 1e-8 * max(1.0, abs(e0)), (e0, e1)
+        out64 = kx.krylov_exp(op, v.to(torch.complex64), 1e-10, 1e-10).to(torch.complex128)
+        fid = abs(torch.vdot(out, out64).item()) ** 2 / float(out64.norm()) ** 2
+        assert 1.0 - fid < 1e-10, 1.0 - fid
+        del piped, plain, out, out64
+        torch.cuda.empty_cache()
