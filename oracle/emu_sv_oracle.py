@@ -12,7 +12,8 @@ purpose (per-qubit ``index_add_`` H.psi, Lanczos with two-vector modified
 Gram-Schmidt, Expokit extended-T error estimate, CPU ``matrix_exp``), because
 it is also the CPU timing baseline.
 
-How each part is pinned (tests/test_oracle_*.py):
+How each part is pinned (tests/test_oracle.py, tests/test_golden.py,
+tests/test_noise_host.py, tests/test_lindblad_host.py):
   * H.psi and the diagonal: dense brute force (``dense_rydberg_hamiltonian``)
     at atol 1e-12, the closed-form energy of (|rrr>+|rgg>)/sqrt2, and the
     reference's seeded energy-variance / second-moment constants.
@@ -31,6 +32,11 @@ How each part is pinned (tests/test_oracle_*.py):
   * expect_batch / apply_1q: dense Kronecker-product operators.
   * Monte-Carlo step: a single decaying atom, whose jump time has the closed
     form exp(-gamma t) = threshold.
+  * Lindbladian: the textbook generator -i[H, rho] + sum (A rho A^H -
+    1/2 {A^H A, rho}) as a dense 4^N superoperator, the reference's no-jump
+    and Tr(H rho) checks, and the e^{-gamma t} decay of one atom.
+  * inverse-CDF sampling: a brute-force scan of the cumulative weights, a
+    point mass, and zero-weight entries that are never drawn.
 """
 
 from __future__ import annotations

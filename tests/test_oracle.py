@@ -295,3 +295,32 @@ def test_midpoint_ramp_convention():
     mid = orc.midpoint_samples(samples, tt, is_amplitude=False)
     k = torch.arange(100, dtype=torch.float64)
     assert torch.allclose(mid, -10 + 20 * (10 * k + 5) / (d - 1), atol=1e-12)
+
+
+def test_inverse_cdf_indices_brute_force():
+    """The oracle's inverse-CDF sampler against a brute-force scan (first index
+    whose cumulative weight exceeds u * total), a point mass, and zero-weight
+    entries that must never be drawn."""
+    g = torch.Generator().manual_seed(11)
+    for size in (1, 2, 7, 64):
+        p = torch.rand(size, dtype=torch.float64, generator=g)
+        if size > 2:
+            p[::3] = 0.0
+        if float(p.sum()) == 0.0:
+            p[-1] = 1.0
+        u = torch.rand(200, dtype=torch.float64, generator=g)
+        got = orc.inverse_cdf_indices(p, u)
+        total = float(p.sum())
+        for k in range(u.numel()):
+            acc, want = 0.0, size - 1
+            for i in range(size):
+                acc += float(p[i])
+                if acc > float(u[k]) * total:
+                    want = i
+                    break
+            assert int(got[k]) == want
+        assert bool((p[got] > 0).all())
+    point = torch.zeros(9, dtype=torch.float64)
+    point[4] = 2.5
+    assert torch.equal(orc.inverse_cdf_indices(point, torch.linspace(0, 0.999, 50, dtype=torch.float64)),
+                       torch.full((50,), 4, dtype=torch.int64))
