@@ -6,15 +6,20 @@ Omega = 2 pi rad/us with sin^2 rise/fall over 200 ns, delta ramp -20 -> 15
 rad/us over 3000 ns, dt = 10 ns (300 Krylov steps), Lanczos tolerance 1e-10,
 initial state |g...g>.  A "step" is one Krylov time step exp(-i dt H) psi.
 
-Timed: ``--steps K`` consecutive steps from t = 0 (K = 300 is the whole
-sequence) after ``--warmup W`` untimed steps, bracketed by device syncs,
-first in complex128 (``value``), then in complex64 with FP64 Lanczos scalars
-(reported with its fidelity against the complex128 run).
+Timed: ``--steps K`` consecutive steps.  When K covers the whole pulse
+(300 steps) one run from t = 0 is timed; otherwise the window is the K steps
+centred on the middle of the pulse: both precisions are first evolved
+(untimed) from t = 0 to the window's first step, and the window is then timed
+``--repeats R`` times (default 5) from that state; ``value`` is from the
+median.  complex128 gives ``value``; complex64 with FP64 Lanczos scalars is
+reported beside it with its fidelity against the complex128 state at the end
+of the window (each precision evolved from t = 0 on its own).
 
-Also measured: the fused Lanczos H.psi (all passes, with its reductions)
-timed with HIP events on the stream it runs on, against HBM peak; and the
-CPU baseline (the oracle, a restatement of the reference's torch-CPU
-algorithm) on a bounded sample of the same workload.
+Also measured: the fused Lanczos iteration (all passes, with its reductions)
+timed with HIP events on the stream it runs on, against HBM peak and against
+a plain copy kernel timed in the same run; and the CPU baseline (the oracle,
+a restatement of the reference's torch-CPU algorithm) running one whole
+Krylov step of the window on the host cores this process may use.
 
 Output: one JSON line on stdout.  Progress goes to stderr.
 """
@@ -51,23 +56,73 @@ def run_steps(seq, psi, k0, count, dtype, tol, stats):
     return psi
 
 
-def timed_run(seq, dtype, steps, warmup, tol):
+def ground_state(n, dtype):
+    import torch
+
+    psi = torch.zeros(2**n, dtype=dtype, device="cuda")
+    psi[0] = 1.0
+    return psi
+
+
+def timed_window(seq, dtype, k0, steps, repeats, warmup, tol):
+    """Evolve untimed from t = 0 to step k0, then time ``steps`` steps from
+    there ``repeats`` times (each from the same state).  Returns the state
+    after the window, the wall times and the matvec count of one window."""
     import torch
 
     from emulators_amd.emu_base.math.krylov_exp import KrylovStats
 
     n = seq.omega.shape[1]
-    psi0 = torch.zeros(2**n, dtype=dtype, device="cuda")
-    psi0[0] = 1.0
+    psi0 = ground_state(n, dtype)
     if warmup:
-        run_steps(seq, psi0.clone(), 0, warmup, dtype, tol, KrylovStats)
+        run_steps(seq, psi0.clone(), k0, min(warmup, steps), dtype, tol, KrylovStats)
+    if k0:
+        log(f"{dtype}: evolving (untimed) to the window's first step {k0}")
+        psi0 = run_steps(seq, psi0, 0, k0, dtype, tol, KrylovStats)
     torch.cuda.synchronize()
-    KrylovStats.reset()
-    t0 = time.perf_counter()
-    psi = run_steps(seq, psi0.clone(), 0, steps, dtype, tol, KrylovStats)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    return psi, wall, KrylovStats.matvecs
+    walls, matvecs, psi = [], None, None
+    for rep in range(repeats):
+        KrylovStats.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        psi = run_steps(seq, psi0.clone(), k0, steps, dtype, tol, KrylovStats)
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+        if matvecs is None:
+            matvecs = KrylovStats.matvecs
+        elif KrylovStats.matvecs != matvecs:
+            log(f"{dtype}: repeat {rep} took {KrylovStats.matvecs} matvecs against {matvecs}")
+        log(f"{dtype}: window repeat {rep + 1}/{repeats} in {walls[-1]:.3f} s")
+    return psi, psi0, walls, matvecs
+
+
+def median(xs):
+    ys = sorted(xs)
+    mid = len(ys) // 2
+    return ys[mid] if len(ys) % 2 else 0.5 * (ys[mid - 1] + ys[mid])
+
+
+def copy_bandwidth(n, reps):
+    """A plain device copy of one complex128 vector (read 16*2^n, write
+    16*2^n bytes), timed with HIP events: the attainable streaming rate."""
+    import torch
+
+    from emulators_amd import _native as native
+
+    src = torch.zeros(2**n, dtype=torch.complex128, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        native.convert(n, src, dst)
+    stream = torch.cuda.current_stream()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(reps):
+        native.convert(n, src, dst)
+    stop.record(stream)
+    stop.synchronize()
+    ms = start.elapsed_time(stop) / reps
+    del src, dst
+    return 2 * 16 * (2**n) / (ms * 1e-3) / 1e9, ms
 
 
 def time_matvec(seq, dtype, reps, step):
@@ -136,41 +191,58 @@ def measured_traffic(tag, sched, n):
     return None
 
 
-def cpu_baseline(seq, mean_iters, step, sample_iters):
-    """Oracle (torch CPU, complex128) on the N=26 Hamiltonian of one step:
-    the diagonal build plus ``sample_iters`` Lanczos iterations, scaled to
-    one Krylov step of ``mean_iters`` iterations."""
+def host_cpus():
+    """CPUs this process may run on: the affinity mask, capped by a cgroup
+    CPU quota when one is set; plus os.cpu_count() and the CPU model."""
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    threads = usable if quota is None else max(1, min(usable, int(quota)))
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads": threads, "os_cpu_count": os.cpu_count(), "affinity": usable,
+            "cgroup_quota_cpus": quota, "model": model}
+
+
+def cpu_baseline(seq, step, state, tol):
+    """One whole Krylov step of the workload on the oracle (torch CPU,
+    complex128, the reference's index_add_ H.psi and Lanczos control flow):
+    step ``step`` from ``state`` (the GPU's state at that step), on every
+    CPU this process may use."""
     import torch
 
     from oracle import emu_sv_oracle as orc
 
-    threads = min(os.cpu_count() or 1, 16)
-    torch.set_num_threads(threads)
-    n = seq.omega.shape[1]
-    dt = 0.01
+    cpus = host_cpus()
+    torch.set_num_threads(cpus["threads"])
+    dt = (seq.target_times[step + 1] - seq.target_times[step]) * 1e-3
+    u = seq.interaction_matrix(0.0)[0]
     t0 = time.perf_counter()
-    ham = orc.OracleHamiltonian(seq.omega[step], seq.delta[step], seq.phi[step],
-                                seq.interaction_matrix(0.0)[0])
+    ham = orc.OracleHamiltonian(seq.omega[step], seq.delta[step], seq.phi[step], u)
     t_diag = time.perf_counter() - t0
-    log(f"CPU baseline: diagonal built in {t_diag:.1f} s")
-    v = torch.zeros(2**n, dtype=torch.complex128)
-    v[0] = 1.0
-    prev = None
-    t0 = time.perf_counter()
-    for _ in range(sample_iters):
-        w = -1j * dt * (ham * v)
-        w.norm()
-        if prev is not None:
-            a = torch.vdot(prev, w)
-            w -= a * prev
-        b = torch.vdot(v, w)
-        w -= b * v
-        nrm = w.norm()
-        w /= nrm
-        prev, v = v, w
-        log(f"CPU baseline: Lanczos iteration done at {time.perf_counter() - t0:.1f} s")
-    t_iter = (time.perf_counter() - t0) / sample_iters
-    t_step = t_diag + mean_iters * t_iter
+    log(f"CPU baseline: diagonal built in {t_diag:.1f} s on {cpus['threads']} threads")
+    v = state.to("cpu", torch.complex128)
+    t1 = time.perf_counter()
+    res = orc.krylov_exp_impl(lambda x: -1j * dt * (ham * x), v, is_hermitian=True,
+                              exp_tolerance=tol, norm_tolerance=tol)
+    t_krylov = time.perf_counter() - t1
+    t_step = t_diag + t_krylov
+    log(f"CPU baseline: Krylov step {step} in {t_step:.1f} s ({res.iteration_count} iterations)")
+    out_state = res.result
+    del res, v, ham
     # the N=10 chain (100 steps) run whole on the oracle: the reference CPU path's own workload
     from emulators_amd import workloads
     seq10 = workloads.chain10()
@@ -181,13 +253,15 @@ def cpu_baseline(seq, mean_iters, step, sample_iters):
     return {
         "value": 1.0 / t_step,
         "unit": "Krylov steps/s",
-        "cores": threads,
+        "cores": cpus["threads"],
         "kind": "port",
-        "sample": (f"oracle (torch-CPU complex128 restatement of the reference) on the N=26 "
-                   f"Hamiltonian of step {step}: diagonal build {t_diag:.2f} s + "
-                   f"{sample_iters} Lanczos iterations at {t_iter:.2f} s each, scaled to "
-                   f"{mean_iters:.1f} iterations per step (the GPU run's mean)"),
+        "sample": (f"oracle (torch-CPU complex128 restatement of the reference) running one whole "
+                   f"Krylov step of the workload, step {step} from the GPU's state at that step: "
+                   f"diagonal build {t_diag:.2f} s + Krylov exponential {t_krylov:.2f} s "
+                   f"({out_state.numel()} amplitudes)"),
+        "host": cpus,
         "chain10_seconds": t_chain10,
+        "_state": out_state,
     }
 
 
@@ -505,7 +579,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--hpsi-reps", type=int, default=20)
-    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--repeats", type=int, default=0,
+                    help="timed repeats of the window (default: 5, or 1 for the whole pulse)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c64", action="store_true")
     ap.add_argument("--duration", type=int, default=3000, help="pulse duration in ns")
@@ -566,20 +641,32 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"copy calibration: {args.hpsi_reps} copies of {16 * 2**n} bytes")
         return 0
-    psi128, wall128, mv128 = timed_run(seq, torch.complex128, steps, args.warmup, tol)
-    log(f"c128: {steps} steps in {wall128:.2f} s, {mv128} matvecs")
+    whole = steps >= nsteps
+    k0 = 0 if whole else (nsteps - steps) // 2
+    repeats = args.repeats if args.repeats else (1 if whole else 5)
+    psi128, start128, walls128, mv128 = timed_window(seq, torch.complex128, k0, steps, repeats,
+                                                     args.warmup, tol)
+    wall128 = median(walls128)
+    log(f"c128: steps {k0}..{k0 + steps - 1}, median {wall128:.3f} s of {repeats}, {mv128} matvecs")
+    start_host = start128.cpu()  # the window's first state, for the CPU leg
+    del start128
     result64 = None
     if not args.no_c64:
-        psi64, wall64, mv64 = timed_run(seq, torch.complex64, steps, args.warmup, tol)
-        log(f"c64: {steps} steps in {wall64:.2f} s, {mv64} matvecs")
+        psi64, start64, walls64, mv64 = timed_window(seq, torch.complex64, k0, steps, repeats,
+                                                     args.warmup, tol)
+        del start64
+        wall64 = median(walls64)
+        log(f"c64: median {wall64:.3f} s of {repeats}, {mv64} matvecs")
         a = psi128
         b = native.convert(n, psi64, torch.empty_like(psi128))
         fid = float(torch.abs(torch.vdot(a, b)) ** 2 / (torch.vdot(a, a).real * torch.vdot(b, b).real))
         occ_a = native.correlation(n, a).diagonal()
         occ_b = native.correlation(n, b).diagonal()
         result64 = {"value": steps / wall64, "ms_per_step": 1e3 * wall64 / steps,
+                    "walls_s": walls64,
                     "speedup_vs_c128": wall128 / wall64, "fidelity_vs_c128": fid,
                     "fidelity_deficit": 1.0 - fid,
+                    "fidelity_over": f"steps 0..{k0 + steps - 1}, each precision from t = 0",
                     "max_occupation_error": float(torch.max(torch.abs(occ_a - occ_b))),
                     "mean_lanczos_iterations": mv64 / steps}
         del psi64, b
@@ -596,6 +683,8 @@ def main() -> int:
         hpsi["c64"] = {"ms": ms64, "alg_bytes": bytes64, "GB/s": bytes64 / (ms64 * 1e-3) / 1e9,
                        "passes": passes64, "schedule": sched64.kwargs()}
     log(f"Lanczos matvec c128 {ms128:.3f} ms ({gbs128:.0f} GB/s algorithmic)")
+    copy_gbs, copy_ms = copy_bandwidth(n, args.hpsi_reps)
+    log(f"copy kernel: {copy_gbs:.0f} GB/s ({copy_ms:.3f} ms per 2 x {16 * 2**n} bytes)")
 
     adjoint = None if args.no_adjoint else adjoint_cost()
     batch = None if args.no_batch else batch_throughput()
@@ -605,8 +694,19 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle) ...")
-        cpu = cpu_baseline(seq, mean_iters, mid, args.cpu_iters)
-        log(f"CPU baseline {cpu['value']:.4f} steps/s on {cpu['cores']} threads")
+        cpu = cpu_baseline(seq, k0, start_host, tol)
+        # the same step on the GPU, for the agreement of the two
+        from emulators_amd.emu_base.math.krylov_exp import KrylovStats
+        gpu_step = run_steps(seq, start_host.to("cuda"), k0, 1, torch.complex128, tol,
+                             KrylovStats).cpu()
+        ref = cpu.pop("_state")
+        cpu["gpu_step_vs_oracle"] = {
+            "step": k0, "max_abs_error": float((gpu_step - ref).abs().max()),
+            "fidelity_deficit": 1.0 - float(torch.abs(torch.vdot(gpu_step, ref)) ** 2 /
+                                            (torch.vdot(gpu_step, gpu_step).real * torch.vdot(ref, ref).real))}
+        del gpu_step, ref
+        log(f"CPU baseline {cpu['value']:.4f} steps/s on {cpu['cores']} threads; GPU step vs oracle "
+            f"max |diff| {cpu['gpu_step_vs_oracle']['max_abs_error']:.1e}")
         if noise is not None:
             cpu["noise"] = cpu_noise_baseline(noise)
     if noise is not None:
@@ -620,6 +720,7 @@ def main() -> int:
         "steps": steps,
         "warmup": args.warmup,
         "ms_per_step": 1e3 * wall128 / steps,
+        "walls_s": walls128,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -631,6 +732,9 @@ def main() -> int:
                         f"{args.duration} ns at dt=10 ns, |g..g>, Krylov tolerance 1e-10",
             "n_qubits": n,
             "time_steps": steps,
+            "window": {"first_step": k0, "last_step": k0 + steps - 1, "of_steps": nsteps,
+                       "repeats": repeats, "statistic": "median" if repeats > 1 else "single run",
+                       "start_state": "evolved untimed from t = 0" if k0 else "|g..g> at t = 0"},
             "mean_lanczos_iterations": mean_iters,
         },
         "roofline": {
@@ -639,6 +743,10 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": gbs128 / HBM_PEAK_GBS,
+            "attainable": copy_gbs,
+            "frac_of_attainable": gbs128 / copy_gbs,
+            "attainable_kernel": "plain device copy of one complex128 vector (emu_sv_convert), "
+                                 "HIP events, same run",
             "traffic": measured_traffic("c128", sched128, n),
             "kernel": f"one Lanczos iteration (emu_sv_lanczos_matvec, complex128): "
                       f"orthogonalise + {passes128} H.psi passes ({sched128.kernel} kernel, "

@@ -150,11 +150,23 @@ def _ptr(t: torch.Tensor | None):
 _RAW_STREAM = getattr(torch._C, "_cuda_getCurrentRawStream", None)
 
 
-def _stream(device: torch.device):
+def _stream_handle(device: torch.device) -> int:
     if _RAW_STREAM is not None:
         idx = device.index if device.index is not None else torch.cuda.current_device()
-        return ctypes.c_void_p(_RAW_STREAM(idx))
-    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+        return int(_RAW_STREAM(idx))
+    return int(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _stream(device: torch.device):
+    return ctypes.c_void_p(_stream_handle(device))
+
+
+def stream_key(device: torch.device) -> tuple[int, int]:
+    """(device index, current stream handle): scratch that kernels update
+    (reduction tickets, partial sums, result slots) is private to one stream,
+    so calls enqueued concurrently on two streams of one device never share it."""
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    return idx, _stream_handle(device)
 
 
 def nqubits_of(vec: torch.Tensor) -> int:
@@ -234,7 +246,10 @@ def plan_passes(n: int, dtype: torch.dtype, tile_log: int = 0, min_contig: int =
 
 
 class Workspace:
-    """Per-device scratch: partial sums plus small result slots."""
+    """Scratch of one (device, stream): partial sums, the reduction ticket,
+    the diagonal table and small result slots.  Every stream that launches
+    the kernels gets its own, so work enqueued concurrently on two streams of
+    one device cannot race on the ticket or the partial rows."""
 
     _cache: dict = {}
 
@@ -243,14 +258,13 @@ class Workspace:
         # zero-filled: holds the reduction ticket counter, which the library re-arms
         self.scratch = torch.zeros(nbytes, dtype=torch.uint8, device=device)
         self.results = torch.zeros(64, dtype=torch.float64, device=device)
-        # the zero fill ran on the current stream; later calls may come from
-        # another stream, so it must be complete before the workspace is shared
+        # the fill is complete before the first kernel of any stream reads it
         torch.cuda.current_stream(device).synchronize()
         self.n = n
 
     @classmethod
     def get(cls, device: torch.device, n: int) -> "Workspace":
-        key = (device.index if device.index is not None else torch.cuda.current_device())
+        key = stream_key(device)
         ws = cls._cache.get(key)
         if ws is None or ws.n < n:
             ws = Workspace(device, max(n, 1))

@@ -20,7 +20,7 @@ LIB_DIR = PKG_DIR / "_lib"
 LIB_NAME = "libemu_sv_hip.so"
 LIB_PATH = LIB_DIR / LIB_NAME
 
-SOURCES = ["emu_sv_kernels.hip", "emu_sv_batched.hip", "emu_sv_host.cpp"]
+SOURCES = ["emu_sv_kernels.hip", "emu_sv_host.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -45,7 +45,10 @@ def hipcc_path() -> str:
 def build(verbose: bool = False) -> Path:
     """Compile every source into the shared library; returns its path."""
     LIB_DIR.mkdir(exist_ok=True)
-    srcs = [str(CSRC / s) for s in SOURCES if (CSRC / s).exists()]
+    missing = [s for s in SOURCES if not (CSRC / s).exists()]
+    if missing:
+        raise RuntimeError(f"missing sources in {CSRC}: {missing}")
+    srcs = [str(CSRC / s) for s in SOURCES]
     tmp = LIB_DIR / (LIB_NAME + ".tmp")
     cmd = [hipcc_path(), *HIPCC_FLAGS, "-I", str(REPO_DIR / "include"), "-o", str(tmp), *srcs]
     if verbose:

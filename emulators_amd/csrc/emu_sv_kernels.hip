@@ -26,6 +26,7 @@
 #include <string.h>
 #include <cmath>
 #include <complex>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -57,8 +58,10 @@ constexpr int kWaves = kThreads / 64;
 constexpr int kMaxTileLog = 14;   // LDS tile <= 2^14 elements
 constexpr int kThreadBits = 8;    // log2(kThreads)
 constexpr int kNumDots = 8;
-constexpr int kDiagRow = 16;  // E_out, a_0 .. a_{t-1} of one pass-1 tile
-static_assert(kDiagRow >= kMaxTileLog + 1, "a diagonal row holds E_out and one a_p per tile bit");
+constexpr int kDiagRow = 16;  // E_out, a_0 .. a_{t-1} of one pass-1 tile, stamp in the last slot
+constexpr int kDiagStamp = kDiagRow - 1;
+static_assert(kDiagStamp >= kMaxTileLog + 1,
+              "a diagonal row holds E_out, one a_p per tile bit and the stamp");
 constexpr int kStreamBlocks = 2048;  // grid of the streaming kernels
 constexpr int kDefaultTileLogC64 = 13;
 constexpr int kDefaultTileLogC128 = 12;
@@ -72,6 +75,7 @@ constexpr int kDefaultContigC128 = 2;
 // Offsets into the packed parameter block (see emu_sv_hip.h).
 constexpr int kCre = 0, kCim = 32, kDelta = 64, kU = 96, kFlagComplex = 1120;
 constexpr int kFlagNoise = 1121, kNoise = 1122;  // K00, K01, K10, K11 (re, im)
+constexpr int kGeneration = 1135;  // pack generation (process-unique, > 0; 0 = unstamped)
 
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
@@ -708,17 +712,33 @@ __global__ __launch_bounds__(kThreads) void hpsi_wave_kernel(const PassArgs a) {
 // outer bits (x >> t), E_out(h) = sum_{q in h} -d_q + sum_{q<q' in h} U_qq'
 // and a_p(h) = -d_p + sum_{q in h} U_pq for the t tile bits.  One thread per
 // table entry (row h, entry p): entry 0 is E_out(h), entry 1 + p is a_p(h),
-// entries beyond t are zero.  The table is written coalesced.
+// entries beyond t are zero, and the last entry is the row's stamp
+// (diag_stamp: the block's pack generation, n and t).  The table is written
+// coalesced.  With keep, a row whose stamp matches is left as it is; rows of
+// an unstamped block (generation 0) are always rebuilt.
+__device__ __forceinline__ double diag_stamp(const double *H, int n, int t) {
+  // exact in FP64 while the generation stays below 2^41
+  return H[kGeneration] * 4096.0 + (double)(n * 64 + t);
+}
+
 __global__ __launch_bounds__(kThreads) void diag_prep_kernel(int n, int t, const double *H,
-                                                             double *table) {
+                                                             double *table, int keep) {
   const uint64_t ntiles = 1ull << (n - t);
   const uint64_t idx = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
   const uint64_t h = idx / kDiagRow;
   const int p = (int)(idx % kDiagRow);
-  if (h >= ntiles) return;
+  const double stamp = diag_stamp(H, n, t);
+  // every row lies inside one block (kThreads is a multiple of kDiagRow):
+  // all its entries read the old stamp before any entry is rewritten
+  const bool fresh = h < ntiles && keep && H[kGeneration] > 0.0 &&
+                     table[h * kDiagRow + kDiagStamp] == stamp;
+  __syncthreads();
+  if (h >= ntiles || fresh) return;
   const uint64_t outer = h << t;
   double e = 0.0;
-  if (p == 0) {
+  if (p == kDiagStamp) {
+    e = stamp;
+  } else if (p == 0) {
     for (int q = t; q < n; ++q) {
       if (!((outer >> q) & 1)) continue;
       e -= H[kDelta + q];
@@ -732,6 +752,7 @@ __global__ __launch_bounds__(kThreads) void diag_prep_kernel(int n, int t, const
   }
   table[idx] = e;
 }
+static_assert(kThreads % kDiagRow == 0, "diag_prep_kernel keeps each row inside one block");
 
 // Two-stage deterministic reduction of nrow rows of kNumDots partial sums:
 // every block sums a fixed contiguous chunk, the last block to finish (ticket
@@ -1775,39 +1796,18 @@ struct OrthoIn {
   NextCoefOut nc;                 // next coefficients formed on the device (optional)
 };
 
-// The diagonal table of the first pass lives in the workspace.  Each
-// workspace remembers (host side) which (parameter block, n, first-pass tile
-// log) its table was last built for; with keep (EMU_SV_FLAG_KEEP_DIAG) a
-// matching table is reused instead of rebuilt.  Different blocks, sizes or
-// tiles always rebuild.
-struct DiagRecord {
-  const double *ham;
-  int n, t1;
-};
-std::mutex g_diag_mu;
-std::unordered_map<const void *, DiagRecord> g_diag_tables;
-
+// The diagonal table of the first pass lives in the workspace.  Its rows
+// carry a stamp of the parameter block's pack generation, n and the
+// first-pass tile log; with keep (EMU_SV_FLAG_KEEP_DIAG) the prep kernel
+// leaves rows whose stamp matches and rebuilds the others, so a table is
+// never reused for a different block, size or tile, whatever address the
+// block has.  Without keep every row is rebuilt.
 int prepare_diag(int n, int t1, const double *ham, void *ws, bool keep, hipStream_t st) {
-  {
-    std::lock_guard<std::mutex> lock(g_diag_mu);
-    auto it = g_diag_tables.find(ws);
-    if (keep && it != g_diag_tables.end() && it->second.ham == ham && it->second.n == n &&
-        it->second.t1 == t1)
-      return EMU_SV_OK;
-    // records of workspaces that were freed are never looked up again; a
-    // bounded map (clearing only forces rebuilds) keeps them from piling up
-    if (it == g_diag_tables.end() && g_diag_tables.size() >= 256) g_diag_tables.clear();
-    g_diag_tables[ws] = DiagRecord{ham, n, t1};
-  }
   const uint64_t ntiles = 1ull << (n - t1);
   hipLaunchKernelGGL(diag_prep_kernel, dim3((unsigned)((ntiles * kDiagRow + kThreads - 1) / kThreads)),
-                     dim3(kThreads), 0, st, n, t1, ham, const_cast<double *>(dtab_ptr(ws, n)));
-  const int rc = hip_check(hipGetLastError(), "diag_prep launch");
-  if (rc) {
-    std::lock_guard<std::mutex> lock(g_diag_mu);
-    g_diag_tables.erase(ws);
-  }
-  return rc;
+                     dim3(kThreads), 0, st, n, t1, ham, const_cast<double *>(dtab_ptr(ws, n)),
+                     keep ? 1 : 0);
+  return hip_check(hipGetLastError(), "diag_prep launch");
 }
 
 template <typename T>
@@ -2420,6 +2420,8 @@ int emu_sv_pack_hamiltonian(int n, const double *omega, const double *delta, con
       block[kU + bj * 32 + bi] = interaction[i * n + j];
     }
   block[kFlagComplex] = cplx ? 1.0 : 0.0;
+  static std::atomic<uint64_t> generation{1};
+  block[kGeneration] = (double)generation.fetch_add(1, std::memory_order_relaxed);
   return EMU_SV_OK;
 }
 

@@ -242,7 +242,7 @@ def _fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
 
 
 class _PipelineBuffers:
-    """Per-device records of the pipelined engine: for iteration j, device
+    """Per-(device, stream) records of the pipelined engine: for iteration j, device
     row j holds [0:12] the reductions, [16:22] the coefficients it reads,
     [24:30] the coefficients' side outputs (n2, b, a, ww of j+1); a pinned
     host mirror receives rows as they complete."""
@@ -257,7 +257,7 @@ class _PipelineBuffers:
 
     @classmethod
     def get(cls, device: torch.device, rows: int) -> "_PipelineBuffers":
-        key = device.index if device.index is not None else torch.cuda.current_device()
+        key = native.stream_key(device)
         b = cls._cache.get(key)
         if b is None or b.rows < rows:
             b = _PipelineBuffers(device, rows)

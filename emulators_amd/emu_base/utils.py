@@ -1,4 +1,10 @@
-"""Logging and memory helpers (reference: emu_base/utils.py:14-135)."""
+"""Logging, memory and readout-error helpers.
+
+Behaviour follows the reference's emu_base/utils.py:14-135: the
+"emulators" logger, the peak-memory probes, and the readout-error
+resampling of a bag of bitstrings (one ``random.random()`` draw per
+character, bitstrings in the bag's order, so a seeded run draws the same
+errors as the reference's)."""
 
 from __future__ import annotations
 
@@ -11,56 +17,59 @@ from pathlib import Path
 
 import torch
 
-unix_like = os.name != "nt"
-if unix_like:
-    from resource import RUSAGE_SELF, getrusage
+unix_like = os.name != "nt"  # platforms with the resource module (public name of emu_base)
+LOGGER_NAME = "emulators"
+_MESSAGE_ONLY = logging.Formatter("%(message)s")
+
+
+def _handler_for(log_file: Path | None) -> logging.Handler:
+    # stdout by default; a log file is truncated on every (re)configuration
+    if log_file is None:
+        return logging.StreamHandler(sys.stdout)
+    return logging.FileHandler(log_file, mode="w")
 
 
 def init_logging(log_level: int, log_file: Path | None) -> logging.Logger:
-    """Configure the "emulators" logger (reference emu_base/utils.py:14-50):
-    no propagation, one handler (stdout or a file opened with mode 'w'),
-    plain "%(message)s" format."""
-    logger = logging.getLogger("emulators")
+    """(Re)configure the "emulators" logger: it passes everything to a single
+    handler of level ``log_level`` and never forwards to the root logger."""
+    logger = logging.getLogger(LOGGER_NAME)
     logger.propagate = False
-    logger.setLevel(logging.DEBUG)
-    handler: logging.Handler
-    if log_file is None:
-        handler = logging.StreamHandler(sys.stdout)
-    else:
-        handler = logging.FileHandler(log_file, mode="w")
+    logger.setLevel(logging.DEBUG)  # the handler's level does the filtering
+    handler = _handler_for(log_file)
     handler.setLevel(log_level)
-    handler.setFormatter(logging.Formatter("%(message)s"))
-    for h in list(logger.handlers):
-        logger.removeHandler(h)
+    handler.setFormatter(_MESSAGE_ONLY)
+    while logger.handlers:
+        logger.removeHandler(logger.handlers[0])
     logger.addHandler(handler)
     return logger
 
 
 def get_max_rss_cpu() -> float:
-    """Peak resident set size in MB."""
-    return getrusage(RUSAGE_SELF).ru_maxrss * 1e-3 if unix_like else 0.0
+    """Peak resident set size of this process in MB (0 where the resource
+    module is unavailable)."""
+    if not unix_like:
+        return 0.0
+    import resource
+
+    return resource.getrusage(resource.RUSAGE_SELF).ru_maxrss * 1e-3
 
 
 def get_max_rss_gpu() -> float:
-    """Peak device memory allocated by torch, in MB, over all devices."""
-    return max(torch.cuda.max_memory_allocated(d) * 1e-6 for d in range(torch.cuda.device_count()))
-
-
-def _readout(c: str, p_false_pos: float, p_false_neg: float) -> str:
-    r = random.random()
-    if c == "0" and r < p_false_pos:
-        return "1"
-    if c == "1" and r < p_false_neg:
-        return "0"
-    return c
+    """Largest peak of torch-allocated device memory over the devices, in MB."""
+    peaks = [torch.cuda.max_memory_allocated(d) for d in range(torch.cuda.device_count())]
+    return max(peaks) * 1e-6
 
 
 def apply_measurement_errors(bitstrings: Counter, *, p_false_pos: float,
                              p_false_neg: float) -> Counter:
-    """Resample a bag of bitstrings with readout errors (reference
-    emu_base/utils.py:102-135)."""
-    result: Counter = Counter()
+    """Resample a bag of bitstrings with readout errors: a '0' reads as '1'
+    with probability ``p_false_pos``, a '1' as '0' with ``p_false_neg``."""
+    threshold = {"0": p_false_pos, "1": p_false_neg}
+    flipped = {"0": "1", "1": "0"}
+    out: Counter = Counter()
     for bits, count in bitstrings.items():
         for _ in range(count):
-            result["".join(_readout(c, p_false_pos, p_false_neg) for c in bits)] += 1
-    return result
+            shot = [flipped[ch] if random.random() < threshold.get(ch, 0.0) else ch
+                    for ch in bits]
+            out["".join(shot)] += 1
+    return out

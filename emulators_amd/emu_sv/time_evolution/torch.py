@@ -165,14 +165,16 @@ class DifferentiableStateVectorEvolution(torch.autograd.Function):
             g_de_np = (fac * (-occ)).real
             g_ph_np = (fac * 0.5 * om.numpy() * (1j * eiph * flips_up - 1j * np.conj(eiph) * flips_dn)).real
             g_u_np = np.triu((fac * pair).real, 1)
+            # each gradient lands on its parameter's device and dtype
             if ctx.needs_input_grad[1]:
-                g_om = torch.as_tensor(g_om_np).to(omegas.dtype)
+                g_om = torch.as_tensor(g_om_np, device=omegas.device).to(omegas.dtype)
             if ctx.needs_input_grad[2]:
-                g_de = torch.as_tensor(g_de_np).to(deltas.dtype)
+                g_de = torch.as_tensor(g_de_np, device=deltas.device).to(deltas.dtype)
             if ctx.needs_input_grad[3]:
-                g_ph = torch.as_tensor(g_ph_np).to(phis.dtype)
+                g_ph = torch.as_tensor(g_ph_np, device=phis.device).to(phis.dtype)
             if ctx.needs_input_grad[4]:
-                g_u = torch.as_tensor(g_u_np).to(interaction_matrix.dtype)
+                g_u = torch.as_tensor(g_u_np, device=interaction_matrix.device).to(
+                    interaction_matrix.dtype)
         if ctx.needs_input_grad[5]:
             g_state = krylov_exp(ScaledHamiltonian(ham, 1j * dt), grad_state_out.detach().clone(),
                                  tol, tol, max_krylov_dim=ctx.max_krylov_dim)

@@ -50,11 +50,13 @@ extern "C" {
 /* The first pass's diagonal constants are tabulated in the workspace by a
  * small kernel before the passes.  With EMU_SV_FLAG_KEEP_DIAG (emu_sv_hpsi,
  * emu_sv_lanczos_matvec, emu_sv_lanczos_matvec_dev) the table is reused when
- * the workspace's last table was built for the same parameter-block pointer,
- * n and first-pass tile size; otherwise it is rebuilt.  The caller asserts
- * that the block's contents have not changed since, and that no call with
- * another n has used the workspace in between.  The Krylov loop of one time
- * step sets it on every iteration after the first. */
+ * it was built from a block of the same pack generation (block slot 1135),
+ * n and first-pass tile size; otherwise it is rebuilt.  The check runs on
+ * the device (one stamp per table row), so a freed block whose address is
+ * reused by another block never reuses a stale table.  A block modified in
+ * place after packing must be repacked (new generation) to be seen as new.
+ * The Krylov loop of one time step sets it on every iteration after the
+ * first. */
 #define EMU_SV_FLAG_KEEP_DIAG 4
 #define EMU_SV_KERNEL_DEFAULT (0 << 8) /* library choice (the wave kernel)  */
 #define EMU_SV_KERNEL_LDS (1 << 8)     /* workgroup tiles staged in LDS     */
@@ -85,7 +87,14 @@ extern "C" {
  *                 (re, im) pairs K00, K01, K10, K11 in the (g, r) basis;
  *                 applied to every qubit: <1|H|0> += K10, <0|H|1> += K01,
  *                 diagonal += K00 per g bit and K11 per r bit
- *                 (emu_sv/hamiltonian.py:61-67, 147-151)             */
+ *                 (emu_sv/hamiltonian.py:61-67, 147-151)
+ *   [1130,1135)   reserved (zero)
+ *   [1135]        pack generation: a process-unique positive integer
+ *                 written by emu_sv_pack_hamiltonian.  EMU_SV_FLAG_KEEP_DIAG
+ *                 reuses a workspace's first-pass diagonal table only when
+ *                 the table was built from a block of the same generation,
+ *                 n and tile (checked on the device); a block of generation
+ *                 0 (not packed by the library) is always rebuilt.      */
 #define EMU_SV_HAM_DOUBLES 1136
 
 /* Last error message of the calling thread ("" if none). */

@@ -1,4 +1,3 @@
-import contextlib
 import random
 import sys
 from pathlib import Path
@@ -16,23 +15,33 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X GPU (runs through the C ABI)")
 
 
-@contextlib.contextmanager
-def fork_all_rng():
-    """Save and restore the python, numpy and torch RNG states."""
-    py_state = random.getstate()
-    np_state = np.random.get_state()
-    try:
-        with torch.random.fork_rng():
-            yield
-    finally:
-        np.random.set_state(np_state)
-        random.setstate(py_state)
+class RngSnapshot:
+    """The state of every random generator a test may draw from: Python's
+    ``random``, numpy's global generator, torch's CPU generator and, once
+    CUDA is initialised, each device generator.  ``restore()`` puts them
+    all back, so seeded tests neither see nor leave draws of other tests."""
+
+    def __init__(self):
+        self.python = random.getstate()
+        self.numpy = np.random.get_state()
+        self.torch_cpu = torch.get_rng_state()
+        self.torch_cuda = (torch.cuda.get_rng_state_all()
+                           if torch.cuda.is_available() and torch.cuda.is_initialized() else None)
+
+    def restore(self) -> None:
+        random.setstate(self.python)
+        np.random.set_state(self.numpy)
+        torch.set_rng_state(self.torch_cpu)
+        if self.torch_cuda is not None:
+            torch.cuda.set_rng_state_all(self.torch_cuda)
 
 
 @pytest.fixture
-def restore_rng_state():
-    with fork_all_rng():
-        yield
+def restore_rng_state(request):
+    """Run the test on its own copy of the random streams."""
+    snap = RngSnapshot()
+    request.addfinalizer(snap.restore)
+    return snap
 
 
 @pytest.fixture(scope="session")

@@ -358,3 +358,96 @@ def test_stream_handle_is_torchs_current_stream(gpu_device):
     side.synchronize()
     ref = _hpsi_gpu(8, om, de, ph, u, psi, torch.complex128)
     assert torch.equal(got, ref)
+
+
+@pytest.mark.parametrize("kernel,tl,cmin,first_log", [("lds", 11, 2, 0), ("tile", 12, 4, 11)])
+def test_keep_diag_ignores_a_reused_block_address(gpu_device, kernel, tl, cmin, first_log):
+    """A parameter block freed and replaced by another block at the same
+    device address: a call with EMU_SV_FLAG_KEEP_DIAG must rebuild the
+    table (the rows carry the pack generation), giving the new block's H."""
+    from emulators_amd import _native as native
+    n = 15
+    kw = dict(tile_log=tl, min_contig=cmin, kernel=kernel, first_log=first_log)
+    psi = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(78))
+    v = psi.to(gpu_device)
+    w = torch.empty_like(v)
+    buf = torch.empty(native.EMU_SV_HAM_DOUBLES, dtype=torch.float64, device=gpu_device)
+    params = []
+    for seed in (8, 9):
+        om, de, ph, u = _params(n, False, seed)
+        block, _ = native.pack_hamiltonian(om.numpy(), de.numpy(), ph.numpy(), u.numpy())
+        params.append(((om, de, ph, u), torch.from_numpy(block)))
+    buf.copy_(params[0][1])
+    native.hpsi(n, buf, 1.0, v, 1.0, w, complex_coef=False, keep_diag=False, **kw)
+    buf.copy_(params[1][1])  # same address, another block (another generation)
+    native.hpsi(n, buf, 1.0, v, 1.0, w, complex_coef=False, keep_diag=True, **kw)
+    om, de, ph, u = params[1][0]
+    ref = OracleHamiltonian(om, de, ph, u) * psi
+    assert float((w.cpu() - ref).norm() / ref.norm()) < 1e-12
+    # a block the library did not pack (generation 0) never reuses a table
+    raw = params[0][1].clone()
+    raw[native.EMU_SV_HAM_DOUBLES - 1] = 0.0
+    buf.copy_(raw)
+    native.hpsi(n, buf, 1.0, v, 1.0, w, complex_coef=False, keep_diag=False, **kw)
+    buf.copy_(params[1][1])
+    buf[native.EMU_SV_HAM_DOUBLES - 1] = 0.0
+    native.hpsi(n, buf, 1.0, v, 1.0, w, complex_coef=False, keep_diag=True, **kw)
+    assert float((w.cpu() - ref).norm() / ref.norm()) < 1e-12
+
+
+def test_concurrent_streams_match_the_oracle(gpu_device, monkeypatch):
+    """Two host threads, each on its own torch stream, run Krylov steps
+    (pipelined Lanczos: fused passes, reductions with tickets, device
+    coefficients) and bare H.psi products at the same time on different
+    states and Hamiltonians.  Each stream has its own workspace, so both
+    results equal the oracle's."""
+    import threading
+
+    from emulators_amd.emu_base.math.krylov_exp import ScaledHamiltonian, krylov_exp
+    from emulators_amd.emu_sv import RydbergHamiltonian
+    from oracle import emu_sv_oracle as orc
+
+    # the tile schedule runs the pipelined engine (device coefficients); fixed
+    # here so that the two threads do not autotune at the same time
+    monkeypatch.setenv("EMU_SV_SCHEDULE", "c128=tile:12:4:0:11")
+    n, dt, tol, steps = 16, 0.01, 1e-10, 6
+    cases = []
+    for seed in (31, 32):
+        om, de, ph, u = _params(n, seed == 32, seed)
+        psi = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(seed))
+        cases.append((om, de, ph, u, psi / psi.norm()))
+    results: dict = {}
+    errors: list = []
+    barrier = threading.Barrier(2)
+
+    def worker(i):
+        try:
+            om, de, ph, u, psi = cases[i]
+            stream = torch.cuda.Stream(device=gpu_device)
+            with torch.cuda.stream(stream):
+                ham = RydbergHamiltonian(om, de, ph, u, device=gpu_device)
+                v = psi.to(gpu_device)
+                barrier.wait()
+                hv = ham * v
+                for _ in range(steps):
+                    v = krylov_exp(ScaledHamiltonian(ham, -1j * dt), v, tol, tol)
+                stream.synchronize()
+                results[i] = (hv.cpu(), v.cpu())
+        except Exception as exc:  # reported by the main thread
+            errors.append((i, repr(exc)))
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not errors, errors
+    for i, (om, de, ph, u, psi) in enumerate(cases):
+        ho = orc.OracleHamiltonian(om, de, ph, u)
+        ref_h = ho * psi
+        ref = psi.clone()
+        for _ in range(steps):
+            ref = orc.krylov_exp(lambda x: -1j * dt * (ho * x), ref.clone(), tol, tol)
+        hv, v = results[i]
+        assert float((hv - ref_h).norm() / ref_h.norm()) < 1e-12, i
+        assert float((v - ref).norm()) < 1e-9, (i, float((v - ref).norm()))

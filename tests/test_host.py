@@ -372,9 +372,29 @@ def test_pack_hamiltonian_steps_matches_single_blocks():
     native.check(native.LIB.emu_sv_pack_hamiltonian_steps(
         n, s, om.ctypes.data_as(dp), de.ctypes.data_as(dp), ph.ctypes.data_as(dp),
         u.ctypes.data_as(dp), blocks.ctypes.data_as(dp)), "pack steps")
+    gen = native.EMU_SV_HAM_DOUBLES - 1
     for k in range(s):
         ref, _ = native.pack_hamiltonian(om[k], de[k], ph[k], u)
-        assert np.array_equal(blocks[k], ref)
+        assert np.array_equal(blocks[k][:gen], ref[:gen])
+    # every packed block gets its own generation (the stamp of the kept diagonal table)
+    gens = [blocks[k][gen] for k in range(s)]
+    assert all(g > 0 and g == int(g) for g in gens) and len(set(gens)) == s
+
+
+def test_pack_generation_is_unique_and_positive():
+    """Slot 1135 of a packed block is a process-unique positive integer, so a
+    diagonal table kept in a workspace is never taken for another block, also
+    when the new block sits at the old one's device address."""
+    from emulators_amd import _native as native
+
+    om, de, ph = np.ones(4), np.zeros(4), np.zeros(4)
+    u = np.zeros((4, 4))
+    gens = [native.pack_hamiltonian(om, de, ph, u)[0][native.EMU_SV_HAM_DOUBLES - 1]
+            for _ in range(5)]
+    assert all(g >= 1 and g == int(g) for g in gens)
+    assert len(set(gens)) == 5 and gens == sorted(gens)
+    block, _ = native.pack_hamiltonian(om, de, ph, u)
+    assert not np.any(block[1130:1135])  # reserved slots stay zero
 
 
 def test_batch_engine_rejects_cpu_and_bad_registers():
