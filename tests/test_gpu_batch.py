@@ -119,3 +119,47 @@ def test_batch_workload_instances_agree(gpu_device):
         ref, _ = _sequential(reg, torch.complex128, 1e-10)
         fid = float(torch.abs(torch.vdot(ref, got.to(torch.complex128))) ** 2)
         assert 1 - fid < 1e-6, (reg.nqubits, 1 - fid)
+
+
+def _oracle_sample(inst, per_n):
+    """Indices of the workload's registers sampled for the oracle check:
+    the first ``per_n[N]`` registers of each size N (in workload order)."""
+    pick, seen = [], {}
+    for i, x in enumerate(inst):
+        n = len(x["positions"])
+        if seen.get(n, 0) < per_n.get(n, 0):
+            pick.append(i)
+            seen[n] = seen.get(n, 0) + 1
+    return pick
+
+
+def test_batch_256_registers_full_workload(gpu_device):
+    """The whole batching workload in one evolve_batch call: 256 seeded
+    registers, N ~ U{8..20}, 500-2000 ns, complex64 with FP64 Lanczos
+    scalars.  No register fails, every final state keeps unit norm to 1e-6,
+    and a seeded sample of 16 registers (N 8..14) matches the oracle's
+    step-by-step Krylov evolution to 1 - F <= 1e-6."""
+    import os
+
+    from emulators_amd import workloads
+    from emulators_amd.emu_sv.batch import evolve_batch, registers_from_sequences
+
+    inst = workloads.batch_instances()
+    assert len(inst) == 256
+    regs = registers_from_sequences([workloads.batch_sequence(x) for x in inst])
+    res = evolve_batch(regs, dtype=torch.complex64)
+    assert res.failed == [], res.failed
+    assert len(res.states) == 256
+    for i, st in enumerate(res.states):
+        nrm = float(st.to(torch.complex128).norm())
+        assert abs(nrm - 1.0) < 1e-6, (i, nrm)
+    torch.set_num_threads(min(16, os.cpu_count() or 1))
+    sample = _oracle_sample(inst, {8: 2, 9: 2, 10: 2, 11: 2, 12: 2, 13: 3, 14: 3})
+    assert len(sample) == 16
+    worst = 0.0
+    for i in sample:
+        ora = _oracle(regs[i], 1e-10)
+        fid = float(orc.fidelity(res.states[i].cpu().to(torch.complex128), ora))
+        worst = max(worst, 1.0 - fid)
+        assert 1.0 - fid <= 1e-6, (i, regs[i].nqubits, 1.0 - fid)
+    print(f"256 registers: 16 sampled against the oracle, worst 1 - F = {worst:.2e}")

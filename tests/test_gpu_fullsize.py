@@ -94,3 +94,48 @@ def test_krylov_step_full_size_properties(gpu_device, step150, monkeypatch, kern
         assert 1.0 - fid < 1e-10, 1.0 - fid
         del piped, plain, out, out64
         torch.cuda.empty_cache()
+
+
+def test_krylov_step_full_size_matches_oracle(gpu_device, step150):
+    """One 1 ns Krylov step of the step-150 Hamiltonian at N=26 from a seeded
+    random state, through the autotuned schedule, against the oracle's
+    krylov_exp_impl (the reference's Lanczos on its index_add_ H.psi):
+    complex128 within 1e-10 of the oracle with the same iteration count, or
+    one apart.  One apart is allowed because the Expokit estimate that stops
+    the iteration is compared with the tolerance: when the estimate lands
+    within rounding of 1e-10, a different summation order (device tree
+    reductions against the oracle's sequential torch sums) can decide the
+    other way; both results are then within the tolerance of exp(-i dt H) psi,
+    so the bound becomes 2e-10.  complex64 (FP64 Lanczos scalars) stays
+    within 1 - F <= 1e-6 of the oracle."""
+    from emulators_amd.emu_base.math.krylov_exp import ScaledHamiltonian, krylov_exp_impl
+    from emulators_amd.emu_sv import RydbergHamiltonian
+
+    om, de, ph, u = step150
+    dt, tol = 0.001, 1e-10
+    torch.set_num_threads(min(16, os.cpu_count() or 1))
+    v = torch.randn(2**N, dtype=torch.complex128, device=gpu_device,
+                    generator=torch.Generator(device=gpu_device).manual_seed(7))
+    v /= v.norm()
+    host = v.cpu()
+    ham = RydbergHamiltonian(om, de, ph, u, device=gpu_device)
+    got = krylov_exp_impl(ScaledHamiltonian(ham, -1j * dt), v.clone(), True, tol, tol)
+    assert got.converged
+    gpu128 = got.result.cpu()
+    m_gpu = got.iteration_count
+    got64 = krylov_exp_impl(ScaledHamiltonian(ham, -1j * dt), v.to(torch.complex64), True, tol, tol)
+    gpu64 = got64.result.cpu().to(torch.complex128)
+    del got, got64, v
+    torch.cuda.empty_cache()
+    ho = orc.OracleHamiltonian(om, de, ph, u)
+    ref = orc.krylov_exp_impl(lambda x: -1j * dt * (ho * x), host, is_hermitian=True,
+                              exp_tolerance=tol, norm_tolerance=tol)
+    assert ref.converged
+    err = float((gpu128 - ref.result).norm())
+    assert abs(m_gpu - ref.iteration_count) <= 1, (m_gpu, ref.iteration_count)
+    bound = tol if m_gpu == ref.iteration_count else 2 * tol
+    assert err <= bound, (err, m_gpu, ref.iteration_count)
+    fid64 = float(orc.fidelity(gpu64, ref.result))
+    assert 1.0 - fid64 <= 1e-6, 1.0 - fid64
+    print(f"N=26 1 ns step: {m_gpu} GPU / {ref.iteration_count} oracle iterations, "
+          f"|c128 - oracle| = {err:.2e}, c64 1 - F = {1 - fid64:.2e}")
