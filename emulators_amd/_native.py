@@ -55,6 +55,7 @@ SIGNATURES = {
     "emu_sv_plan_passes": (_i, [_i, _i, _i, _i, _i]),
     "emu_sv_workspace_bytes": (ctypes.c_size_t, [_i, _i]),
     "emu_sv_hpsi": (_i, [_i, _i, _vp, _d, _d, _vp, _d, _vp, _d, _vp, _vp, _vp, _i, _i, _i, _vp]),
+    "emu_sv_hpsi_static": (_i, [_i, _i, _vp, _d, _d, _vp, _d, _vp, _d, _vp, _vp, _i, _i, _i, _vp]),
     "emu_sv_lanczos_update": (_i, [_i, _i, _vp, _vp, _d, _d, _d, _vp, _d, _d, _d, _vp, _vp, _vp]),
     "emu_sv_lanczos_matvec": (_i, [_i, _i, _vp, _d, _d, _vp, _d, _vp, _d, _d, _d, _vp, _d, _d, _d,
                                    _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),

@@ -2386,6 +2386,59 @@ int lanczos_matvec_dev_impl(int n, int dtype, const double *ham, double sigma_re
                          tile_log, min_contig, tile_partners(flags), cplx, stream, &o, keep);
 }
 
+int hpsi_impl(int n, int dtype, const double *ham, double sigma_re, double sigma_im, const void *v,
+              double v_scale, const void *prev, double prev_scale, void *w, double *dots_out,
+              void *workspace, int tile_log, int min_contig, int flags, hipStream_t stream);
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Static scratch of emu_sv_hpsi_static: the workspace of callers that cannot
+// allocate device memory (code under stream capture, tight serving loops).
+// Each GPU holds its own copy of the array until the process ends.  In host
+// code the array's name is only a stand-in for the symbol: its device
+// address is looked up once per device (hipGetSymbolAddress, with that
+// device current) and only that address is handed to kernels.  Every call
+// on a GPU shares the array: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr size_t kStaticScratchBytes = 24ull << 20;  // the workspace of n <= 24
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_static_mu;
+void *g_static_ptr[kMaxDevices] = {};
+
+// Clears the reduction ticket and the finalize stage (the only parts of a
+// workspace that must start at zero; the ticket is re-armed by every
+// reduction).  A fill kernel rather than a memset, so the first call may
+// also be recorded into a graph.
+__global__ void zero_words_kernel(uint32_t *p, size_t words) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < words;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+int static_scratch(hipStream_t st, void **out) {
+  int dev = 0;
+  if (int rc = hip_check(hipGetDevice(&dev), "hipGetDevice")) return rc;
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(EMU_SV_EINVAL, "hpsi_static: device index above " + std::to_string(kMaxDevices));
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  if (!g_static_ptr[dev]) {
+    void *p = nullptr;
+    if (int rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                           "hipGetSymbolAddress(static scratch)"))
+      return rc;
+    const size_t words = ws_layout(1).partials / sizeof(uint32_t);  // ticket + stage
+    hipLaunchKernelGGL(zero_words_kernel, dim3(8), dim3(kThreads), 0, st,
+                       reinterpret_cast<uint32_t *>(p), words);
+    if (int rc = hip_check(hipGetLastError(), "static scratch clear")) return rc;
+    g_static_ptr[dev] = p;
+  }
+  *out = g_static_ptr[dev];
+  return EMU_SV_OK;
+}
 }  // namespace
 
 // ===========================================================================
@@ -2465,11 +2518,38 @@ int emu_sv_hpsi(int n, int dtype, const double *ham, double sigma_re, double sig
                 const void *v, double v_scale, const void *prev, double prev_scale, void *w,
                 double *dots_out, void *workspace, int tile_log, int min_contig, int flags,
                 hipStream_t stream) {
+  if (!workspace) return fail(EMU_SV_EINVAL, "hpsi: null workspace (it holds the diagonal table)");
+  return hpsi_impl(n, dtype, ham, sigma_re, sigma_im, v, v_scale, prev, prev_scale, w, dots_out,
+                   workspace, tile_log, min_contig, flags, stream);
+}
+
+int emu_sv_hpsi_static(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
+                       const void *v, double v_scale, const void *prev, double prev_scale, void *w,
+                       double *dots_out, int tile_log, int min_contig, int flags,
+                       hipStream_t stream) {
+  if (int rc = check_n(n)) return rc;
+  const size_t need = ws_layout(n).total;
+  if (need > kStaticScratchBytes)
+    return fail(EMU_SV_EINVAL, "hpsi_static: n = " + std::to_string(n) + " needs " +
+                                   std::to_string(need) + " bytes of scratch, above the library's " +
+                                   std::to_string(kStaticScratchBytes) +
+                                   "-byte static array (n <= 24); use emu_sv_hpsi with a workspace");
+  void *ws = nullptr;
+  if (int rc = static_scratch(stream, &ws)) return rc;
+  return hpsi_impl(n, dtype, ham, sigma_re, sigma_im, v, v_scale, prev, prev_scale, w, dots_out, ws,
+                   tile_log, min_contig, flags, stream);
+}
+
+}  // extern "C"
+
+namespace {
+int hpsi_impl(int n, int dtype, const double *ham, double sigma_re, double sigma_im, const void *v,
+              double v_scale, const void *prev, double prev_scale, void *w, double *dots_out,
+              void *workspace, int tile_log, int min_contig, int flags, hipStream_t stream) {
   if (int rc = check_n(n)) return rc;
   if (int rc = check_dtype(dtype)) return rc;
   if (!ham || !v || !w) return fail(EMU_SV_EINVAL, "hpsi: null pointer");
   if (v == w) return fail(EMU_SV_EINVAL, "hpsi: input and output must not alias");
-  if (!workspace) return fail(EMU_SV_EINVAL, "hpsi: null workspace (it holds the diagonal table)");
   int kind = kernel_kind(flags);
   if (kind != kKernelLds && kind != kKernelWave && kind != kKernelTile)
     return fail(EMU_SV_EINVAL, "hpsi: unknown kernel kind in flags");
@@ -2515,6 +2595,9 @@ int emu_sv_hpsi(int n, int dtype, const double *ham, double sigma_re, double sig
   return run_hpsi<float>(n, ham, sigma_re, sigma_im, v, v_scale, prev, prev_scale, w, dots_out,
                          workspace, kind, tile_log, min_contig, cplx, stream, nullptr, noise, keep);
 }
+}  // namespace
+
+extern "C" {
 
 int emu_sv_lanczos_matvec(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
                           void *w, double w_scale, const void *pa, double pa_scale, double a_re,

@@ -153,6 +153,23 @@ int emu_sv_hpsi(int n, int dtype, const double *ham, double sigma_re,
                 void *workspace, int tile_log, int min_contig, int flags,
                 hipStream_t stream);
 
+/* emu_sv_hpsi without a workspace argument: it allocates nothing and uses
+ * instead a static device array of the library (24 MiB per GPU, held until
+ * the process ends), so it serves n <= 24 and returns EMU_SV_EINVAL beyond.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The first call on a device looks up the array's
+ * address with that device current and clears its reduction ticket with a
+ * fill kernel on `stream`.  Every call on a GPU shares that GPU's array and
+ * it keeps the last call's contents: calls that overlap on one GPU (two
+ * streams, or two threads) race.  Same arguments and results as
+ * emu_sv_hpsi otherwise (the same kernels). */
+int emu_sv_hpsi_static(int n, int dtype, const double *ham, double sigma_re,
+                       double sigma_im, const void *v, double v_scale,
+                       const void *prev, double prev_scale, void *w,
+                       double *dots_out, int tile_log, int min_contig, int flags,
+                       hipStream_t stream);
+
 /* One fused Lanczos iteration step (orthogonalise, then the next matvec):
  *   z = w_scale*w - a*(pa_scale*pa) - b*(pb_scale*pb)   (stored back into w)
  *   r = sigma*H*z
