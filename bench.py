@@ -640,6 +640,8 @@ def main() -> int:
             native.convert(n, src, dst)
         torch.cuda.synchronize()
         log(f"copy calibration: {args.hpsi_reps} copies of {16 * 2**n} bytes")
+        gbs, ms = copy_bandwidth(n, args.hpsi_reps)
+        log(f"copy kernel: {gbs:.0f} GB/s ({ms:.3f} ms per 2 x {16 * 2**n} bytes)")
         return 0
     whole = steps >= nsteps
     k0 = 0 if whole else (nsteps - steps) // 2

@@ -183,6 +183,8 @@ def _require_device_vec(vec: torch.Tensor, what: str) -> None:
         raise ValueError(f"{what}: the emu_sv engine needs device (GPU) tensors, got {vec.device}")
     if not vec.is_contiguous():
         raise ValueError(f"{what}: tensor must be contiguous")
+    if vec.data_ptr() % 16:
+        raise ValueError(f"{what}: vectors must be 16-byte aligned (the kernels move 16-byte packets)")
 
 
 def _require_doubles(t: torch.Tensor, count: int, what: str) -> None:

@@ -1336,18 +1336,23 @@ __global__ __launch_bounds__(kThreads) void sample_draw_kernel(uint64_t D, uint6
   const int lane = threadIdx.x & 63;
   const int k = blockIdx.x * kWaves + (threadIdx.x >> 6);
   if (k >= count) return;
+  // The reference's test, searchsorted(cdf / total, u, right=True): the first
+  // index whose normalised cumulative weight exceeds u.  Normalising each
+  // cumulative sum (a correctly rounded division, monotone in the sum)
+  // rather than scaling u makes every decision equal to the reference's
+  // whenever the cumulative sums themselves are exact.
   const double total = ends[nchunks - 1];
-  const double target = uniforms[k] * total;
-  // first chunk whose inclusive end exceeds the target
+  const double u = uniforms[k];
+  // first chunk whose normalised inclusive end exceeds u
   uint64_t lo = 0, hi = nchunks;
   while (lo < hi) {
     const uint64_t mid = (lo + hi) / 2;
-    if (ends[mid] > target) hi = mid;
+    if (ends[mid] / total > u) hi = mid;
     else lo = mid + 1;
   }
   if (lo >= nchunks) lo = nchunks - 1;
   const uint64_t c = lo;
-  const double r = target - (c > 0 ? ends[c - 1] : 0.0);
+  const double base = c > 0 ? ends[c - 1] : 0.0;
   uint64_t x0, x1;
   sample_lane_range(D, c, lane, x0, x1);
   double s = 0.0;
@@ -1359,15 +1364,15 @@ __global__ __launch_bounds__(kThreads) void sample_draw_kernel(uint64_t D, uint6
     const double o = __shfl_up(pre, off, 64);
     if (lane >= off) pre += o;
   }
-  const unsigned long long hit = __ballot(pre > r);
+  const unsigned long long hit = __ballot((base + pre) / total > u);
   int64_t found = -1;
   if (hit) {
     const int l = __ffsll((long long)hit) - 1;
     if (lane == l) {
-      double run = pre - s;
+      double run = base + (pre - s);
       for (uint64_t x = x0; x < x1; ++x) {
         run += sample_prob<T>(src, stride, mode, x);
-        if (run > r) {
+        if (run / total > u) {
           found = (int64_t)x;
           break;
         }
@@ -1376,8 +1381,8 @@ __global__ __launch_bounds__(kThreads) void sample_draw_kernel(uint64_t D, uint6
     }
     found = __shfl((long long)found, l, 64);
   } else {
-    // rounding left r at or above the chunk's own sum: its last element
-    // with non-zero probability
+    // rounding kept every normalised sum of the chunk at or below u: the
+    // chunk's last element with non-zero probability
     uint64_t cx0, cx1;
     sample_lane_range(D, c, 0, cx0, cx1);
     const uint64_t chunk = D < (1ull << kSampleChunkLog) ? D : (1ull << kSampleChunkLog);
@@ -1403,6 +1408,26 @@ __global__ __launch_bounds__(kThreads) void convert_kernel(uint64_t D,
     o.x = static_cast<D2>(z.x);
     o.y = static_cast<D2>(z.y);
     dst[x] = o;
+  }
+}
+
+// Same-type copy in 16-byte packets: each thread issues its four loads
+// (a block-stride apart, so every load instruction is coalesced) before its
+// four stores; one pass, no grid-stride loop.  npk is a multiple of
+// 4 * kThreads or the tail is guarded.
+__global__ __launch_bounds__(kThreads) void copy_packets_kernel(uint64_t npk, const uint4 *src,
+                                                                uint4 *dst) {
+  const uint64_t base = (uint64_t)blockIdx.x * (4 * kThreads) + threadIdx.x;
+  uint4 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint64_t i = base + (uint64_t)k * kThreads;
+    if (i < npk) v[k] = src[i];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint64_t i = base + (uint64_t)k * kThreads;
+    if (i < npk) dst[i] = v[k];
   }
 }
 
@@ -2951,12 +2976,12 @@ int emu_sv_convert(int n, int src_dtype, const void *src, int dst_dtype, void *d
   else if (src_dtype == EMU_SV_C64 && dst_dtype == EMU_SV_C128)
     hipLaunchKernelGGL((convert_kernel<float, double>), dim3(nb), dim3(kThreads), 0, stream, D,
                        (const float2 *)src, (double2 *)dst);
-  else if (src_dtype == EMU_SV_C128)
-    hipLaunchKernelGGL((convert_kernel<double, double>), dim3(nb), dim3(kThreads), 0, stream, D,
-                       (const double2 *)src, (double2 *)dst);
-  else
-    hipLaunchKernelGGL((convert_kernel<float, float>), dim3(nb), dim3(kThreads), 0, stream, D,
-                       (const float2 *)src, (float2 *)dst);
+  else {
+    const uint64_t npk = packets(n, src_dtype);
+    const uint64_t per = 4ull * kThreads;
+    hipLaunchKernelGGL(copy_packets_kernel, dim3((unsigned)((npk + per - 1) / per)), dim3(kThreads), 0,
+                       stream, npk, (const uint4 *)src, (uint4 *)dst);
+  }
   return hip_check(hipGetLastError(), "convert launch");
 }
 

@@ -15,6 +15,9 @@ dtype = torch.complex128
 DenseStateType = TypeVar("DenseStateType", bound="DenseState")
 
 
+SAMPLE_MAX_CATEGORIES = 2**24  # torch.multinomial limit (emu_sv/utils.py:38)
+
+
 class DenseState:
     """A complex128 vector over the qubit basis; index bit N-1-q is qubit q,
     bit value 1 = "r" (dense_state.py:183-185)."""
@@ -37,21 +40,28 @@ class DenseState:
                p_false_neg: float = 0.0) -> Counter:
         """Bitstring counts drawn from |psi|^2 or |diag(rho)| (dense_state.py:40-95).
 
-        On the GPU the indices come from the native inverse-CDF sampler
-        (emu_sv_sample) fed with torch.rand draws; on the CPU from
-        ``sample_indices``.  Either way the distribution is the reference's;
-        seeded draws differ from torch.multinomial's."""
-        if self.data.is_cuda:
-            from emulators_amd import _native as native
-            u = torch.rand(num_shots, dtype=torch.float64, device=self.data.device)
-            outcomes = native.sample(self.n_qudits, self.data.contiguous(), u,
-                                     density_matrix=self.data.dim() == 2).cpu()
+        Up to 2^24 categories the indices come from torch.multinomial, as in
+        the reference (emu_sv/utils.py:38-39), so seeded counts are the
+        reference's.  Beyond, on the GPU, the native inverse-CDF sampler
+        (emu_sv_sample) turns the reference's torch.rand draws into indices
+        with the reference's test, searchsorted(cdf / total, u, right=True)
+        (utils.py:41-49); on the CPU ``sample_indices`` does."""
+        density = self.data.dim() == 2
+        categories = self.data.shape[0]
+        if categories <= SAMPLE_MAX_CATEGORIES or not self.data.is_cuda:
+            probabilities = torch.abs(self.data.diagonal()) if density else torch.abs(self.data) ** 2
+            outcomes = sample_indices(probabilities, num_shots, SAMPLE_MAX_CATEGORIES).cpu()
         else:
-            if self.data.dim() == 2:
-                probabilities = torch.abs(self.data.diagonal())
-            else:
-                probabilities = torch.abs(self.data) ** 2
-            outcomes = sample_indices(probabilities, num_shots)
+            from emulators_amd import _native as native
+            data = self.data.contiguous()
+            n = self.n_qudits
+            total = (float(torch.abs(data.diagonal()).sum()) if density
+                     else float(native.dot(n, data.view(-1))[0].item()))
+            if not total > 0:
+                # same exception and message as torch.multinomial
+                raise RuntimeError("invalid multinomial distribution (sum of probabilities <= 0)")
+            u = torch.rand(num_shots, dtype=torch.float64, device=data.device)
+            outcomes = native.sample(n, data, u, density_matrix=density).cpu()
         counts = Counter(index_to_bitstring(self.n_qudits, int(o)) for o in outcomes.tolist())
         if p_false_neg > 0 or p_false_pos > 0:
             counts = apply_measurement_errors(counts, p_false_pos=p_false_pos,

@@ -10,7 +10,8 @@
  * Conventions shared with the reference emulator (paths relative to the
  * reference checkout):
  *   - state: 1-D contiguous complex vector of length D = 2^N, interleaved
- *     (re, im); complex128 ("c128") or complex64 ("c64").
+ *     (re, im); complex128 ("c128") or complex64 ("c64"); every device
+ *     vector is 16-byte aligned (kernels move 16-byte packets).
  *   - qubit q <-> bit (N-1-q) of the amplitude index: qubit 0 is the most
  *     significant bit (emu_sv/hamiltonian.py:107); bit value 1 = Rydberg "r".
  *   - H = sum_q (O_q/2)[cos(phi_q) X_q + sin(phi_q) Y_q] - sum_q d_q n_q

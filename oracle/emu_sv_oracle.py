@@ -819,9 +819,13 @@ def dense_lindblad_superoperator(omegas, deltas, phis, interaction_matrix, lindb
 # Sampling  (emu_sv/utils.py:17-49, emu_sv/dense_state.py:40-95)
 # ---------------------------------------------------------------------------
 def inverse_cdf_indices(probabilities: torch.Tensor, uniforms: torch.Tensor) -> torch.Tensor:
-    """searchsorted(cdf, u * total, right=True) clamped to the last index: the
-    inverse-CDF branch of sample_indices (utils.py:36-49), with the draws
-    given."""
+    """The inverse-CDF branch of sample_indices (emu_sv/utils.py:41-49) with
+    the draws given: cdf = cumsum in float64, cdf /= cdf[-1], then
+    searchsorted(cdf, u, right=True) clamped to the last index."""
     cdf = torch.cumsum(probabilities.to(torch.float64), dim=0)
-    idx = torch.searchsorted(cdf, uniforms.to(torch.float64) * cdf[-1], right=True)
+    total = cdf[-1].item()
+    if not total > 0:
+        raise RuntimeError("invalid multinomial distribution (sum of probabilities <= 0)")
+    cdf /= total
+    idx = torch.searchsorted(cdf, uniforms.to(torch.float64), right=True)
     return idx.clamp_(max=probabilities.numel() - 1)
