@@ -29,8 +29,9 @@ EMU_SV_KERNEL_DEFAULT = 0
 EMU_SV_KERNEL_LDS = 1 << 8
 EMU_SV_KERNEL_WAVE = 2 << 8
 EMU_SV_KERNEL_TILE = 3 << 8
+EMU_SV_KERNEL_BTILE = 4 << 8
 KERNELS = {"default": EMU_SV_KERNEL_DEFAULT, "lds": EMU_SV_KERNEL_LDS, "wave": EMU_SV_KERNEL_WAVE,
-           "tile": EMU_SV_KERNEL_TILE}
+           "tile": EMU_SV_KERNEL_TILE, "btile": EMU_SV_KERNEL_BTILE}
 
 
 def kernel_flags(kernel: str, partners: int = 0, first_log: int = 0) -> int:

@@ -5,6 +5,10 @@ A schedule is (kernel kind, tile_log, min_contig):
 * "wave": a wavefront owns 2^tile_log amplitudes (tile_log = 6 lane bits +
   3..5 register bits); flips are register or cross-lane operations.
 * "lds":  a 256-thread workgroup stages 2^tile_log amplitudes in LDS.
+* "btile": 1024-thread workgroups holding 2^14 (c128) / 2^15 (c64)
+  amplitudes in registers (wave-bit flips through LDS): a contiguous first
+  pass and later passes of ``min_contig`` low bits plus high bits; two passes
+  cover n = 26 (``tile_log`` is ignored).
 * "tile": a contiguous first pass (orthogonalisation + diagonal) and later
   passes of ``min_contig`` contiguous low bits plus high bits, the last of
   which reads the flip partners of up to ``partners`` top bits directly
@@ -55,12 +59,14 @@ CANDIDATES = {
                       Schedule("tile", 13, 5, 0, 11), Schedule("tile", 13, 6, 0, 12),
                       Schedule("tile", 13, 4, 4, 13), Schedule("tile", 12, 4, 6, 12),
                       Schedule("tile", 14, 5, 4, 13), Schedule("tile", 14, 5, 6, 12),
-                      Schedule("tile", 14, 6, 6, 12), Schedule("tile", 13, 5, 6, 12)],
+                      Schedule("tile", 14, 6, 6, 12), Schedule("tile", 13, 5, 6, 12),
+                      Schedule("btile", 0, 4), Schedule("btile", 0, 5)],
     torch.complex128: [Schedule("lds", 10, 2), Schedule("lds", 11, 2), Schedule("wave", 10, 2),
                        Schedule("tile", 12, 4, 0, 10), Schedule("tile", 12, 4, 0, 11),
                        Schedule("tile", 12, 5, 0, 10), Schedule("tile", 13, 5, 0, 10),
                        Schedule("tile", 12, 4, 6, 12), Schedule("tile", 13, 4, 4, 13),
-                       Schedule("tile", 13, 4, 6, 12), Schedule("tile", 13, 5, 6, 12)],
+                       Schedule("tile", 13, 4, 6, 12), Schedule("tile", 13, 5, 6, 12),
+                       Schedule("btile", 0, 2), Schedule("btile", 0, 4)],
 }
 
 _cache: dict = {}

@@ -1706,6 +1706,7 @@ __global__ __launch_bounds__(kThreads) void traces_finalize_kernel(int n, int gx
 }
 
 #include "emu_sv_tile.inc"
+#include "emu_sv_btile.inc"
 #include "emu_sv_batch.inc"
 
 // ---------------------------------------------------------------------------
@@ -2082,6 +2083,124 @@ int run_tile(int n, const double *ham, double sr, double si, const void *v, doub
 
 
 // ---------------------------------------------------------------------------
+// Big-tile passes (btile_kernel): A = contiguous [0, TB) with the diagonal
+// (and the orthogonalisation), then B passes of c + m = TB tile bits with
+// c >= cmin contiguous low bits; at n <= 2 TB - cmin one B pass covers the
+// rest.  TB = 14 (c128) or 15 (c64); needs n > TB.
+// ---------------------------------------------------------------------------
+int bt_tile_bits_of(int dtype) { return dtype == EMU_SV_C64 ? 15 : 14; }
+
+int make_btile_plan(int n, int tb, int cmin, Pass *out) {
+  int np = 0;
+  out[np++] = {tb, tb, 0, bits_mask(0, tb), 1};
+  int next = tb;
+  while (next < n) {
+    const int rem = n - next;
+    const int m = rem < tb - cmin ? rem : tb - cmin;
+    out[np++] = {tb - m, next, m, bits_mask(next, next + m), 0};
+    next += m;
+  }
+  return np;
+}
+
+// 0: valid; 1: n not above one tile (use the LDS kernel); < 0: error.
+int check_btile_schedule(int n, int dtype, int &cmin, const char *what) {
+  const int tb = bt_tile_bits_of(dtype);
+  if (cmin <= 0) cmin = dtype == EMU_SV_C64 ? 4 : 2;
+  if (cmin < 1 || cmin >= tb)
+    return fail(EMU_SV_EINVAL, std::string(what) + ": btile min_contig must be in [1, " +
+                                   std::to_string(tb) + ")");
+  return n <= tb ? 1 : 0;
+}
+
+template <typename T, bool CPLX, bool MODEB, bool DOTS>
+int launch_btile_one(const BtArgs &a, hipStream_t st) {
+  static std::once_flag once;
+  static hipError_t attr = hipSuccess;
+  const size_t lds = (size_t)kBtRoundSlots * kBtThreads * 16;
+  std::call_once(once, [&] {
+    attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&btile_kernel<T, CPLX, MODEB, DOTS>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  });
+  if (int rc = hip_check(attr, "hipFuncSetAttribute(btile kernel LDS)")) return rc;
+  const uint64_t nt = 1ull << (a.n - bt_tile_bits<T>());
+  hipLaunchKernelGGL((btile_kernel<T, CPLX, MODEB, DOTS>), dim3((unsigned)nt), dim3(kBtThreads), lds,
+                     st, a);
+  return hip_check(hipGetLastError(), "btile kernel launch");
+}
+
+template <typename T, bool CPLX>
+int launch_btile_c(const BtArgs &a, bool modeb, bool dots, hipStream_t st) {
+  if (!modeb) return launch_btile_one<T, CPLX, false, false>(a, st);
+  return dots ? launch_btile_one<T, CPLX, true, true>(a, st) : launch_btile_one<T, CPLX, true, false>(a, st);
+}
+
+template <typename T>
+int launch_btile(const BtArgs &a, bool cplx, bool modeb, bool dots, hipStream_t st) {
+  return cplx ? launch_btile_c<T, true>(a, modeb, dots, st) : launch_btile_c<T, false>(a, modeb, dots, st);
+}
+
+template <typename T>
+int run_btile(int n, const double *ham, double sr, double si, const void *v, double vsc, void *w,
+              double *dots, void *ws, int cmin, bool cplx, hipStream_t st, const OrthoIn *ortho) {
+  constexpr int TB = bt_tile_bits<T>();
+  Pass plan[EMU_SV_MAX_QUBITS];
+  const int np = make_btile_plan(n, TB, cmin, plan);
+  int nrow_a = 0;
+  for (int p = 0; p < np; ++p) {
+    const bool last = p == np - 1;
+    BtArgs a{};
+    a.v = v;  // with orthogonalisation, pass A has overwritten v with z
+    a.acc = w;
+    a.out = w;
+    a.ham = ham;
+    a.partials = partials_ptr(ws, n);
+    a.partials2 = partials2_ptr(ws, n);
+    a.ortho = (ortho != nullptr && p == 0) ? 1 : 0;
+    a.zout = const_cast<void *>(v);
+    a.pa = ortho ? ortho->pa : nullptr;
+    a.pb = ortho ? ortho->pb : nullptr;
+    a.ca_re = ortho ? ortho->ca_re : 0.0;
+    a.ca_im = ortho ? ortho->ca_im : 0.0;
+    a.cb_re = ortho ? ortho->cb_re : 0.0;
+    a.cb_im = ortho ? ortho->cb_im : 0.0;
+    a.pb_scale = ortho ? ortho->pb_scale : 0.0;
+    a.dcoef = (ortho != nullptr && p == 0) ? ortho->dcoef : nullptr;
+    a.v_scale = (ortho != nullptr && p > 0) ? 1.0 : vsc;  // later passes read z = v_scale*w already
+    a.sig_re = sr;
+    a.sig_im = si;
+    a.n = n;
+    a.c = plan[p].c;
+    a.s = plan[p].s;
+    a.m = plan[p].m;
+    a.flip_mask = plan[p].fmask;
+    const bool want_dots = last && dots != nullptr;
+    if (want_dots && p == 0)
+      return fail(EMU_SV_EINVAL, "btile: the reductions need a second pass (n above one tile)");
+    int rc = launch_btile<T>(a, cplx, p > 0, want_dots, st);
+    if (rc) return rc;
+    const int nrow = (int)(1ull << (n - TB));
+    if (a.ortho) {
+      if (dots) {
+        nrow_a = nrow;  // reduced with the last pass's dots
+      } else {
+        rc = finalize(a.partials2, nrow, kOrthoSums, ws, n, ortho->norm2_out, st);
+        if (rc) return rc;
+      }
+    }
+    if (want_dots) {
+      // the fused dots of a btile pass: [0] ||w||^2, [1,2] <v|w>; [3..6] zero
+      rc = ortho ? finalize_pair(a.partials2, nrow_a, kOrthoSums, ortho->norm2_out, a.partials, nrow,
+                                 7, dots, ws, n, ortho->nc, st)
+                 : finalize(a.partials, nrow, 7, ws, n, dots, st);
+      if (rc) return rc;
+    }
+  }
+  return EMU_SV_OK;
+}
+
+
+// ---------------------------------------------------------------------------
 // Batch engine host side.
 // ---------------------------------------------------------------------------
 struct BatchHandle {
@@ -2371,10 +2490,29 @@ int lanczos_matvec_dev_impl(int n, int dtype, const double *ham, double sigma_re
   if (w == r || w == pa || w == pb || r == pa || r == pb)
     return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: w, r, pa and pb must be distinct buffers");
   const int kind = kernel_kind(flags);
-  if (kind != kKernelTile && kind != kKernelLds)
-    return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: needs the tile or LDS kernel kind");
+  if (kind != kKernelTile && kind != kKernelLds && kind != kKernelBtile)
+    return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: needs the tile, btile or LDS kernel kind");
   if (flags & EMU_SV_FLAG_NOISE)
     return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: the Lanczos recurrence needs a Hermitian H (no noise term)");
+  if (kind == kKernelBtile) {
+    const int rc = check_btile_schedule(n, dtype, min_contig, "lanczos_matvec_dev");
+    if (rc < 0) return rc;
+    if (rc > 0) return fail(EMU_SV_EINVAL, "lanczos_matvec_dev: btile needs more qubits than one tile");
+    OrthoIn o;
+    o.pa = pa;
+    o.pb = pb;
+    o.ca_re = o.ca_im = o.cb_re = o.cb_im = 0.0;
+    o.pb_scale = 0.0;
+    o.norm2_out = norm2_out;
+    o.dcoef = coef;
+    o.nc = nc;
+    const bool bc = (flags & EMU_SV_FLAG_COMPLEX) != 0;
+    if (dtype == EMU_SV_C128)
+      return run_btile<double>(n, ham, sigma_re, sigma_im, w, 1.0, r, dots_out, workspace, min_contig,
+                               bc, stream, &o);
+    return run_btile<float>(n, ham, sigma_re, sigma_im, w, 1.0, r, dots_out, workspace, min_contig, bc,
+                            stream, &o);
+  }
   int tla = 0;
   if (kind == kKernelTile) {
     tla = tile_first_log(flags, 0);
@@ -2516,6 +2654,17 @@ int emu_sv_pack_noise(const double *noise, double *block) {
 int emu_sv_plan_passes(int n, int dtype, int tile_log, int min_contig, int flags) {
   if (int rc = check_n(n)) return rc;
   if (int rc = check_dtype(dtype)) return rc;
+  if (kernel_kind(flags) == kKernelBtile) {
+    const int rc = check_btile_schedule(n, dtype, min_contig, "plan_passes");
+    if (rc < 0) return rc;
+    if (rc == 0) {
+      Pass bplan[EMU_SV_MAX_QUBITS];
+      return make_btile_plan(n, bt_tile_bits_of(dtype), min_contig, bplan);
+    }
+    tile_log = 0;
+    min_contig = 0;
+    flags = EMU_SV_KERNEL_LDS;
+  }
   if (kernel_kind(flags) == kKernelTile) {
     int tla = tile_first_log(flags, 0);
     const int rc = check_tile_schedule(n, dtype, tile_log, min_contig, tla, "plan_passes");
@@ -2576,11 +2725,26 @@ int hpsi_impl(int n, int dtype, const double *ham, double sigma_re, double sigma
   if (!ham || !v || !w) return fail(EMU_SV_EINVAL, "hpsi: null pointer");
   if (v == w) return fail(EMU_SV_EINVAL, "hpsi: input and output must not alias");
   int kind = kernel_kind(flags);
-  if (kind != kKernelLds && kind != kKernelWave && kind != kKernelTile)
+  if (kind != kKernelLds && kind != kKernelWave && kind != kKernelTile && kind != kKernelBtile)
     return fail(EMU_SV_EINVAL, "hpsi: unknown kernel kind in flags");
   const bool noise = (flags & EMU_SV_FLAG_NOISE) != 0;
   const bool cplx = noise || (flags & EMU_SV_FLAG_COMPLEX) != 0;
   const bool keep = (flags & EMU_SV_FLAG_KEEP_DIAG) != 0;
+  if (kind == kKernelBtile && !noise) {
+    const int rc = check_btile_schedule(n, dtype, min_contig, "hpsi");
+    if (rc < 0) return rc;
+    if (rc == 0 && !prev) {  // the btile passes do not reduce the prev dots
+      if (!workspace) return fail(EMU_SV_EINVAL, "hpsi: null workspace");
+      if (dtype == EMU_SV_C128)
+        return run_btile<double>(n, ham, sigma_re, sigma_im, v, v_scale, w, dots_out, workspace,
+                                 min_contig, cplx, stream, nullptr);
+      return run_btile<float>(n, ham, sigma_re, sigma_im, v, v_scale, w, dots_out, workspace,
+                              min_contig, cplx, stream, nullptr);
+    }
+    kind = kKernelLds;  // n within one tile, or prev dots wanted: the LDS kernel
+    tile_log = 0;
+    min_contig = 0;
+  }
   if (noise && kind != kKernelLds) {  // only the LDS kernel carries the noise term
     kind = kKernelLds;
     tile_log = 0;
@@ -2636,10 +2800,34 @@ int emu_sv_lanczos_matvec(int n, int dtype, const double *ham, double sigma_re, 
   if (w == r || w == pa || w == pb || r == pa || r == pb)
     return fail(EMU_SV_EINVAL, "lanczos_matvec: w, r, pa and pb must be distinct buffers");
   int kind = kernel_kind(flags);
-  if (kind != kKernelLds && kind != kKernelWave && kind != kKernelTile)
+  if (kind != kKernelLds && kind != kKernelWave && kind != kKernelTile && kind != kKernelBtile)
     return fail(EMU_SV_EINVAL, "lanczos_matvec: unknown kernel kind in flags");
   if (flags & EMU_SV_FLAG_NOISE)
     return fail(EMU_SV_EINVAL, "lanczos_matvec: the Lanczos recurrence needs a Hermitian H (no noise term)");
+  if (kind == kKernelBtile) {
+    const int rc = check_btile_schedule(n, dtype, min_contig, "lanczos_matvec");
+    if (rc < 0) return rc;
+    if (rc == 0) {
+      OrthoIn o;
+      o.pa = pa;
+      o.pb = pb;
+      o.ca_re = a_re * pa_scale;
+      o.ca_im = a_im * pa_scale;
+      o.cb_re = b_re * pb_scale;
+      o.cb_im = b_im * pb_scale;
+      o.pb_scale = pb_scale;
+      o.norm2_out = norm2_out;
+      const bool bc = (flags & EMU_SV_FLAG_COMPLEX) != 0;
+      if (dtype == EMU_SV_C128)
+        return run_btile<double>(n, ham, sigma_re, sigma_im, w, w_scale, r, dots_out, workspace,
+                                 min_contig, bc, stream, &o);
+      return run_btile<float>(n, ham, sigma_re, sigma_im, w, w_scale, r, dots_out, workspace,
+                              min_contig, bc, stream, &o);
+    }
+    kind = kKernelLds;
+    tile_log = 0;
+    min_contig = 0;
+  }
   bool use_tile = false;
   int tla = kind == kKernelTile ? tile_first_log(flags, 0) : 0;
   if (kind == kKernelTile) {

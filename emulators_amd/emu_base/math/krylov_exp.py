@@ -394,9 +394,12 @@ def fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
     n = native.nqubits_of(v)
     if n == ham.nqubits:
         sched = autotune.best_schedule(n, v.dtype, v.device, ham.device_params(v.device), ham.complex)
-        tl = sched.tile_log or (13 if v.dtype == torch.complex64 else 12)
-        fl = sched.first_log or tl
-        if PIPELINE and sched.kernel in ("tile", "lds") and n >= max(tl, fl):
+        if sched.kernel == "btile":  # one big tile: 2^15 (c64) / 2^14 (c128) amplitudes
+            tl = fl = 16 if v.dtype == torch.complex64 else 15
+        else:
+            tl = sched.tile_log or (13 if v.dtype == torch.complex64 else 12)
+            fl = sched.first_log or tl
+        if PIPELINE and sched.kernel in ("tile", "lds", "btile") and n >= max(tl, fl):
             return _pipelined_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim, sched,
                                       rows)
     return _fused_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim, rows)

@@ -67,6 +67,12 @@ extern "C" {
                                        /* of c contiguous low bits (min_contig) */
                                        /* and the next high bits; tile_log  */
                                        /* 11..13 (c128), 11..14 (c64)        */
+#define EMU_SV_KERNEL_BTILE (4 << 8)   /* big register tiles (1024 threads, */
+                                       /* 2^14 c128 / 2^15 c64 amplitudes): */
+                                       /* a contiguous first pass, then     */
+                                       /* passes of min_contig low bits and */
+                                       /* the next high bits (tile_log is   */
+                                       /* ignored); two passes up to n = 26 */
 /* Tile kernel only: the last pass reads the flip partners of up to P
  * leftover top bits directly (orbit-ordered tiles; 2, 4 or 6 of them are
  * compiled), P in flags bits 12-15; the first (contiguous) pass may use its

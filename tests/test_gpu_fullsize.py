@@ -18,8 +18,10 @@ from oracle import emu_sv_oracle as orc
 pytestmark = pytest.mark.gpu
 N = 26
 SCHEDULES = {
-    torch.complex128: [("lds", 11, 2, 0, 0), ("tile", 12, 4, 0, 11), ("tile", 13, 6, 0, 12)],
-    torch.complex64: [("lds", 11, 3, 0, 0), ("tile", 13, 6, 0, 12), ("tile", 12, 5, 0, 12)],
+    torch.complex128: [("lds", 11, 2, 0, 0), ("tile", 12, 4, 0, 11), ("tile", 13, 6, 0, 12),
+                       ("btile", 0, 2, 0, 0)],
+    torch.complex64: [("lds", 11, 3, 0, 0), ("tile", 13, 6, 0, 12), ("tile", 12, 5, 0, 12),
+                      ("btile", 0, 4, 0, 0)],
 }
 
 
