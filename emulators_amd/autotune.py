@@ -146,6 +146,95 @@ def best_schedule(n: int, dtype: torch.dtype, device: torch.device, params: torc
     return sched
 
 
+# Batch engine: (tile_log, min_contig) candidates of the LDS pass kernel per
+# element type, and the tuned choice per (device, dtype, atom count).
+BATCH_CANDIDATES = {
+    torch.complex64: [(11, 3), (12, 4), (13, 4), (13, 5), (14, 5)],
+    torch.complex128: [(10, 2), (11, 2), (12, 3), (13, 4)],
+}
+_batch_cache: dict = {}
+_batch_timings: dict = {}
+
+
+def _truncated(registers, steps: int):
+    from emulators_amd.emu_sv.batch import BatchRegister
+
+    return [BatchRegister(r.omegas[:steps], r.deltas[:steps], r.phis[:steps], r.interaction_matrix,
+                          r.dts[:steps], r.psi0) for r in registers]
+
+
+def _time_batch(regs, picks: dict, dtype, device, tolerance, max_krylov_dim, reps: int = 2) -> float:
+    """Best device seconds of ``reps`` batched runs of ``regs`` with schedule
+    picks[n] per atom count (set-up and graph capture are not timed)."""
+    from dataclasses import replace
+
+    from emulators_amd.emu_sv.batch import evolve_batch
+
+    sched = [replace(r, tile_log=picks[r.nqubits][0], min_contig=picks[r.nqubits][1]) for r in regs]
+    return min(evolve_batch(sched, dtype=dtype, tolerance=tolerance, max_krylov_dim=max_krylov_dim,
+                            device=device).seconds for _ in range(reps))
+
+
+def tune_batch(registers, dtype: torch.dtype, device: torch.device, tolerance: float,
+               max_krylov_dim: int, steps: int = 4, gain: float = 0.02, refine: int = 4) -> dict:
+    """(tile_log, min_contig) per atom count for this set of registers, timed
+    on the set itself: every register truncated to its first ``steps``
+    steps, run as one batch.  First the best uniform schedule, then, for the
+    atom counts in decreasing order (the largest carry most of the work),
+    each alternative schedule kept when it saves more than ``gain`` of the
+    batch time (the ``refine`` largest counts only: the work per register
+    grows as 2^N).  Atom counts of at most 10 (2^10 amplitudes: one tile of
+    any candidate) keep the library default."""
+    sizes = sorted({r.nqubits for r in registers}, reverse=True)
+    regs = _truncated(registers, steps)
+    cands = BATCH_CANDIDATES[dtype]
+    small = {n: (0, 0) for n in sizes if n <= 10}
+    best_t, best = None, None
+    table = {}
+    for tl, cmin in cands:
+        picks = {**{n: (tl, cmin) for n in sizes}, **small}
+        t = _time_batch(regs, picks, dtype, device, tolerance, max_krylov_dim)
+        table[("uniform", tl, cmin)] = t
+        if best_t is None or t < best_t:
+            best_t, best = t, picks
+    for n in [x for x in sizes if x > 10][:refine]:
+        for alt in cands:
+            if alt == best[n]:
+                continue
+            picks = {**best, n: alt}
+            t = _time_batch(regs, picks, dtype, device, tolerance, max_krylov_dim)
+            table[(n, alt[0], alt[1])] = t
+            if t < best_t * (1.0 - gain):
+                best_t, best = t, picks
+    return {"picks": best, "seconds": best_t, "table": table}
+
+
+def with_batch_schedules(registers, dtype: torch.dtype, device: torch.device, tolerance: float,
+                         max_krylov_dim: int) -> list:
+    """Copies of the registers, each without a schedule of its own given the
+    tuned one for its atom count (cached per device, dtype and set of sizes)."""
+    from dataclasses import replace
+
+    if not _enabled():
+        return list(registers)
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    key = (idx, dtype, tuple(sorted(r.nqubits for r in registers)))
+    res = _batch_cache.get(key)
+    if res is None:
+        res = tune_batch(registers, dtype, device, tolerance, max_krylov_dim)
+        _batch_cache[key] = res
+        _batch_timings[key] = res["table"]
+    picks = res["picks"]
+    return [r if (r.tile_log or r.min_contig) else replace(r, tile_log=picks[r.nqubits][0],
+                                                            min_contig=picks[r.nqubits][1])
+            for r in registers]
+
+
+def batch_tuned_table() -> dict:
+    """{(device, dtype, sizes): {candidate: seconds}} of every batch tuning so far."""
+    return dict(_batch_timings)
+
+
 def tuned_table() -> dict:
     """{(device, n, dtype, complex): {schedule: ms}} of every tuning done so far."""
     return dict(_timings)

@@ -2232,6 +2232,14 @@ int batch_plan(const emu_sv_batch_register &r, int dtype, int tl, int cmin, Batc
   if ((reinterpret_cast<uintptr_t>(r.psi) | reinterpret_cast<uintptr_t>(r.psi_alt) |
        reinterpret_cast<uintptr_t>(r.rows)) & 15)
     return fail(EMU_SV_EINVAL, "batch: psi, psi_alt and rows must be 16-byte aligned");
+  // a register may carry its own schedule in its flags (0: the batch's)
+  const int rtl = (r.flags >> 16) & 0xF, rcmin = (r.flags >> 20) & 0xF;
+  if (rtl) tl = rtl;
+  if (rcmin) cmin = rcmin;
+  const int tmax = dtype == EMU_SV_C64 ? 14 : 13;  // 128 KiB of LDS
+  if (tl < kThreadBits || tl > tmax || cmin < 1 || cmin >= tl)
+    return fail(EMU_SV_EINVAL, "batch: register schedule needs tile_log in [8, 13] (complex64: 14) "
+                               "and min_contig in [1, tile_log)");
   const int t = r.n < tl ? r.n : tl;
   Pass plan[EMU_SV_MAX_QUBITS];
   const int np = make_plan(r.n, t, cmin, plan);
@@ -2248,7 +2256,6 @@ int batch_plan(const emu_sv_batch_register &r, int dtype, int tl, int cmin, Batc
     out.tiles[p] = 1ull << (r.n - (plan[p].c + plan[p].m));
     if (out.tiles[p] > out.ntiles_max) out.ntiles_max = out.tiles[p];
   }
-  (void)dtype;
   return EMU_SV_OK;
 }
 

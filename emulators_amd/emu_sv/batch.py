@@ -38,6 +38,8 @@ class BatchRegister:
     interaction_matrix: np.ndarray
     dts: np.ndarray
     psi0: torch.Tensor | None = None
+    tile_log: int = 0    # this register's H.psi schedule (0: the batch's)
+    min_contig: int = 0
 
     @property
     def nqubits(self) -> int:
@@ -83,12 +85,22 @@ def _pack_steps(reg: BatchRegister) -> tuple[np.ndarray, bool]:
 def evolve_batch(registers: list[BatchRegister], dtype: torch.dtype = torch.complex64,
                  tolerance: float = 1e-10, max_krylov_dim: int = 60, tile_log: int = 0,
                  min_contig: int = 0, check_every: int = 32, max_iterations: int = 1_000_000,
-                 device: torch.device | str = "cuda") -> BatchResult:
+                 device: torch.device | str = "cuda", tiles: str | None = None) -> BatchResult:
     """Evolve every register through all its steps; returns the final states
-    (device tensors of ``dtype``) and per-register matvec counts."""
+    (device tensors of ``dtype``) and per-register matvec counts.
+
+    ``tiles="auto"``: each register without a schedule of its own takes the
+    one the autotuner picked for its atom count on this register set
+    (autotune.with_batch_schedules)."""
     device = torch.device(device)
     if device.type != "cuda":
         raise ValueError("the batch engine runs on the GPU")
+    if tiles == "auto":
+        from emulators_amd import autotune
+        registers = autotune.with_batch_schedules(registers, dtype, device, tolerance,
+                                                  max_krylov_dim)
+    elif tiles is not None:
+        raise ValueError("evolve_batch: tiles must be None or 'auto'")
     code = native.dtype_code(dtype)
     count = len(registers)
     if count < 1:
@@ -114,8 +126,11 @@ def evolve_batch(registers: list[BatchRegister], dtype: torch.dtype = torch.comp
         rows = torch.empty((max_krylov_dim + 1) * 2**n, dtype=dtype, device=device)
         keep += [ham, dts, alt, rows]
         states.append(psi)
-        regs[i] = _Reg(n, reg.steps, native.EMU_SV_FLAG_COMPLEX if cplx else 0, 0, ham.data_ptr(),
-                       dts.data_ptr(), psi.data_ptr(), alt.data_ptr(), rows.data_ptr())
+        if not (0 <= reg.tile_log <= 15 and 0 <= reg.min_contig <= 15):
+            raise ValueError("batch register: tile_log and min_contig must be in [0, 15]")
+        flags = (native.EMU_SV_FLAG_COMPLEX if cplx else 0) | (reg.tile_log << 16) | (reg.min_contig << 20)
+        regs[i] = _Reg(n, reg.steps, flags, 0, ham.data_ptr(), dts.data_ptr(), psi.data_ptr(),
+                       alt.data_ptr(), rows.data_ptr())
     nbytes = native.LIB.emu_sv_batch_workspace_bytes(count, code, regs, tile_log, min_contig)
     if nbytes == 0:
         raise ValueError("evolve_batch: invalid register set or tile schedule")

@@ -353,10 +353,15 @@ int emu_sv_parameter_traces(int n, int dtype, int count, const void *w,
 /* One register of a batch.  All pointers are device memory owned by the
  * caller and 16-byte aligned.  psi holds the initial state and receives the
  * final state. */
+#define EMU_SV_BATCH_TILE_LOG(t) (((t) & 0xF) << 16)
+#define EMU_SV_BATCH_MIN_CONTIG(c) (((c) & 0xF) << 20)
 typedef struct emu_sv_batch_register {
   int n;              /* qubits (1..30) */
   int steps;          /* time steps */
-  int flags;          /* EMU_SV_FLAG_COMPLEX if any step has phi != 0 */
+  int flags;          /* EMU_SV_FLAG_COMPLEX if any step has phi != 0; optionally */
+                      /* this register's own H.psi schedule:                   */
+                      /* EMU_SV_BATCH_TILE_LOG(t) | EMU_SV_BATCH_MIN_CONTIG(c)  */
+                      /* (0: the batch's tile_log / min_contig)                */
   int pause_each_step; /* 1: stop after every step until emu_sv_batch_resume */
   const double *ham;  /* steps * EMU_SV_HAM_DOUBLES packed blocks, one per step */
   const double *dt;   /* steps time steps in us (op = -i dt H) */
