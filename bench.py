@@ -172,6 +172,15 @@ def time_matvec(seq, dtype, reps, step):
     return ms, alg_bytes, passes, sched
 
 
+def tile_text(sched, btile_log):
+    """The tile size of a schedule, for the bench line."""
+    if sched.kernel == "btile":
+        return f"2^{btile_log}-amplitude tiles in registers, >= {sched.min_contig} contiguous bits in pass B"
+    if sched.kernel == "default" or sched.tile_log == 0:
+        return "library default tiles"
+    return f"tile 2^{sched.tile_log}, >= {sched.min_contig} contiguous bits"
+
+
 def measured_traffic(tag, sched, n):
     """HBM bytes per Lanczos iteration from the committed rocprofv3 counter
     runs, when one was taken on this schedule and size (else None)."""
@@ -764,8 +773,9 @@ def main() -> int:
             "traffic": measured_traffic("c128", sched128, n),
             "kernel": f"one Lanczos iteration (emu_sv_lanczos_matvec, complex128): "
                       f"orthogonalise + {passes128} H.psi passes ({sched128.kernel} kernel, "
-                      f"tile 2^{sched128.tile_log}) + reductions; algorithmic bytes 3*D*16 per call "
+                      f"{tile_text(sched128, 14)}) + reductions; algorithmic bytes 3*D*16 per call "
                       "(read v_j, v_{j-1}, write v_{j+1})",
+            "schedule": sched128.kwargs(),
         },
         "cpu_baseline": cpu,
         "c64": result64,
