@@ -2314,7 +2314,10 @@ int batch_enqueue_iteration(BatchHandle &h, hipStream_t st) {
   if (int rc = hip_check(hipGetLastError(), "batch decide launch")) return rc;
   hipLaunchKernelGGL(batch_combine_kernel<T>, dim3((unsigned)h.ncomb), dim3(kThreads), 0, st, h.regs,
                      h.comb);
-  return hip_check(hipGetLastError(), "batch combine launch");
+  if (int rc = hip_check(hipGetLastError(), "batch combine launch")) return rc;
+  hipLaunchKernelGGL(batch_advance_kernel, dim3((unsigned)((h.count + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                     st, h.regs, h.count);
+  return hip_check(hipGetLastError(), "batch advance launch");
 }
 
 // Workspace layout (bytes): [partial rows | pass-1 diagonal table |
@@ -3314,7 +3317,6 @@ int emu_sv_batch_create(int count, int dtype, const emu_sv_batch_register *regs,
     R.j = 0;
     R.phase = kPhStart;
     R.m = 0;
-    R.ticket = 0;
     R.matvecs = 0;
     double dt0 = 0.0;
     if (int rc = hip_check(hipMemcpy(&dt0, U.dt, sizeof(double), hipMemcpyDeviceToHost),
