@@ -71,6 +71,10 @@ SIGNATURES = {
     "emu_sv_pack_noise": (_i, [_dp, _dp]),
     "emu_sv_arnoldi_orthogonalize": (_i, [_i, _i, _i, ctypes.POINTER(_vp), _dp, _vp, _vp, _vp,
                                           _vp]),
+    "emu_sv_arnoldi_orthonormalize": (_i, [_i, _i, _i, ctypes.POINTER(_vp), _dp, _vp, _vp, _vp,
+                                           _vp]),
+    "emu_sv_arnoldi_exp": (_i, [_i, _i, _vp, _d, _d, _vp, _vp, _vp, _i, _d, _d, _vp, _vp, _vp, _i, _i,
+                                _i, _vp, ctypes.POINTER(_i)]),
     "emu_sv_apply_1q": (_i, [_i, _i, _vp, _i, _dp, _vp]),
     "emu_sv_apply_pair": (_i, [_i, _i, _vp, _vp, _i, _i, _dp, _i, _vp]),
     "emu_sv_antihermitian": (_i, [_i, _i, _vp, _vp, _d, _d, _vp]),
@@ -426,11 +430,12 @@ def dot(n: int, a: torch.Tensor, b: torch.Tensor | None = None) -> torch.Tensor:
 
 
 def arnoldi_orthogonalize(n: int, basis: list[torch.Tensor], scales: list[float],
-                          w: torch.Tensor) -> torch.Tensor:
+                          w: torch.Tensor, normalize: bool = False) -> torch.Tensor:
     """Modified Gram-Schmidt of w against scales[k] * basis[k] on the device.
     Returns a device (count+1, 4) float64 tensor: row k < count =
     [||w||^2 before subtraction k, Re, Im <basis_k|w>] (unscaled), row count =
-    [||w||^2 after the last subtraction]."""
+    [||w||^2 after the last subtraction].  ``normalize``: w is then divided
+    by that norm on the device (the next orthonormal basis vector)."""
     _require_device_vec(w, "arnoldi_orthogonalize")
     count = len(basis)
     if count != len(scales) or count < 1:
@@ -446,11 +451,55 @@ def arnoldi_orthogonalize(n: int, basis: list[torch.Tensor], scales: list[float]
     rec = torch.empty(count + 1, 4, dtype=torch.float64, device=w.device)
     arr = (ctypes.c_void_p * count)(*[v.data_ptr() for v in basis])
     sc = np.ascontiguousarray(np.asarray(scales, dtype=np.float64))
-    check(LIB.emu_sv_arnoldi_orthogonalize(n, dtype_code(w.dtype), count, arr,
-                                           sc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                           _ptr(w), _ptr(rec), _ptr(ws.scratch),
-                                           _stream(w.device)), "arnoldi_orthogonalize")
+    fn = LIB.emu_sv_arnoldi_orthonormalize if normalize else LIB.emu_sv_arnoldi_orthogonalize
+    check(fn(n, dtype_code(w.dtype), count, arr, sc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+             _ptr(w), _ptr(rec), _ptr(ws.scratch), _stream(w.device)), "arnoldi_orthogonalize")
     return rec
+
+
+class _ArnoldiExpBuffers:
+    """Per-(device, stream, n, dtype, max_dim) buffers of emu_sv_arnoldi_exp:
+    the basis, the device records and their pinned host mirror."""
+
+    _cache: dict = {}
+
+    def __init__(self, device, n: int, dtype, max_dim: int):
+        rows = (max_dim * (max_dim + 2) + 1) * 4
+        self.basis = torch.empty(max_dim << n, dtype=dtype, device=device)
+        self.records = torch.zeros(rows, dtype=torch.float64, device=device)
+        self.host = torch.zeros(rows, dtype=torch.float64, pin_memory=True)
+
+    @classmethod
+    def get(cls, device, n: int, dtype, max_dim: int) -> "_ArnoldiExpBuffers":
+        key = (stream_key(device), n, dtype, max_dim)
+        b = cls._cache.get(key)
+        if b is None:
+            b = cls._cache[key] = cls(device, n, dtype, max_dim)
+        return b
+
+
+def arnoldi_exp(n: int, ham_dev: torch.Tensor, sigma: complex, v: torch.Tensor, out: torch.Tensor,
+                max_dim: int, exp_tolerance: float, norm_tolerance: float, complex_coef: bool = True,
+                noise: bool = False, tile_log: int = 0, min_contig: int = 0, kernel: str = "default",
+                partners: int = 0, first_log: int = 0) -> tuple[int, bool, bool, int]:
+    """out = exp(sigma H) v by the library's pipelined Arnoldi driver; returns
+    (iterations, converged, happy breakdown, matvecs)."""
+    _require_device_vec(v, "arnoldi_exp")
+    _require_device_vec(out, "arnoldi_exp")
+    if v.numel() != 1 << n or out.numel() != 1 << n or out.dtype != v.dtype:
+        raise ValueError("arnoldi_exp: v and out must be 2**n elements of one dtype")
+    bufs = _ArnoldiExpBuffers.get(v.device, n, v.dtype, max_dim)
+    ws = Workspace.get(v.device, n)
+    flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags(kernel, partners, first_log)
+    if noise:
+        flags |= EMU_SV_FLAG_NOISE
+    res = (ctypes.c_int * 4)()
+    check(LIB.emu_sv_arnoldi_exp(n, dtype_code(v.dtype), _ptr(ham_dev), float(sigma.real), float(sigma.imag),
+                                 _ptr(v), _ptr(out), _ptr(bufs.basis), max_dim, float(exp_tolerance),
+                                 float(norm_tolerance), _ptr(bufs.records), _ptr(bufs.host),
+                                 _ptr(ws.scratch), tile_log, min_contig, flags, _stream(v.device), res),
+          "arnoldi_exp")
+    return int(res[0]), bool(res[1]), bool(res[2]), int(res[3])
 
 
 def apply_1q(n: int, psi: torch.Tensor, qubit: int, op) -> torch.Tensor:

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <atomic>
@@ -992,6 +993,7 @@ __global__ __launch_bounds__(kThreads) void arnoldi_mgs_kernel(uint64_t npk, voi
 // the cost is the per-step reduction: four waves (eight at D = 2^13) beat
 // sixteen (n = 10: 15 us against 33 us; n = 12: 27 us against 41 us).
 constexpr int kMgsSmallLog = 13;
+constexpr int kArnoldiMaxDim = 256;  // emu_sv_arnoldi_exp: Krylov dimension limit
 constexpr int kMgsSmallCount = 128;
 struct MgsSmallArgs {
   const void *u[kMgsSmallCount];
@@ -1021,7 +1023,7 @@ __device__ __forceinline__ void mgs_block_sum(double (&v)[K], double *red) {
 template <typename T, int EPT, int NT>
 __global__ __launch_bounds__(NT) void arnoldi_mgs_small_kernel(int count, void *wv,
                                                                const MgsSmallArgs args,
-                                                               double *records) {
+                                                               double *records, int normalize) {
   using T2 = typename Vec2<T>::type;
   __shared__ double red[(NT / 64) * 3];
   T2 *w = reinterpret_cast<T2 *>(wv);
@@ -1068,28 +1070,49 @@ __global__ __launch_bounds__(NT) void arnoldi_mgs_small_kernel(int count, void *
   for (int e = 0; e < EPT; ++e) nrm[0] += wr[e] * wr[e] + wi[e] * wi[e];
   mgs_block_sum<NT, 1>(nrm, red);
   if (threadIdx.x == 0) records[4 * count] = nrm[0];
+  // normalize: store the unit vector (the next basis vector), rounded once
+  const double sc = (normalize && nrm[0] > 0.0) ? 1.0 / sqrt(nrm[0]) : 1.0;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     T2 o;
-    o.x = static_cast<T>(wr[e]);
-    o.y = static_cast<T>(wi[e]);
+    o.x = static_cast<T>(wr[e] * sc);
+    o.y = static_cast<T>(wi[e] * sc);
     w[threadIdx.x + e * NT] = o;
+  }
+}
+
+// w <- w / sqrt(*norm2) in place (left as it is when the norm is zero): the
+// normalisation of the launch-chain Arnoldi step, from its device record.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void scale_by_norm_kernel(uint64_t npk, void *wv, const double *norm2) {
+  using P = typename Packet<T>::type;
+  P *v = reinterpret_cast<P *>(wv);
+  const double nn = *norm2;
+  if (!(nn > 0.0)) return;
+  const double s = 1.0 / sqrt(nn);
+  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  for (uint64_t x = (uint64_t)blockIdx.x * kThreads + threadIdx.x; x < npk; x += stride) {
+    P z = v[x];
+#pragma unroll
+    for (int k = 0; k < Packet<T>::amps; ++k)
+      packet_set(z, k, static_cast<double>(packet_re(z, k)) * s, static_cast<double>(packet_im(z, k)) * s);
+    v[x] = z;
   }
 }
 
 template <typename T>
 int launch_mgs_small(int n, int count, const void *const *vecs, const double *scales, void *w,
-                     double *records, hipStream_t st) {
+                     double *records, int normalize, hipStream_t st) {
   MgsSmallArgs args;
   for (int k = 0; k < count; ++k) {
     args.u[k] = vecs[k];
     args.s2[k] = scales[k] * scales[k];
   }
   switch (n) {
-    case 10: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 4, 256>), dim3(1), dim3(256), 0, st, count, w, args, records); break;
-    case 11: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 8, 256>), dim3(1), dim3(256), 0, st, count, w, args, records); break;
-    case 12: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 16, 256>), dim3(1), dim3(256), 0, st, count, w, args, records); break;
-    case 13: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 16, 512>), dim3(1), dim3(512), 0, st, count, w, args, records); break;
+    case 10: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 4, 256>), dim3(1), dim3(256), 0, st, count, w, args, records, normalize); break;
+    case 11: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 8, 256>), dim3(1), dim3(256), 0, st, count, w, args, records, normalize); break;
+    case 12: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 16, 256>), dim3(1), dim3(256), 0, st, count, w, args, records, normalize); break;
+    case 13: hipLaunchKernelGGL((arnoldi_mgs_small_kernel<T, 16, 512>), dim3(1), dim3(512), 0, st, count, w, args, records, normalize); break;
     default: return fail(EMU_SV_EINVAL, "arnoldi_orthogonalize: small path needs 10 <= n <= 13");
   }
   return hip_check(hipGetLastError(), "arnoldi_mgs_small launch");
@@ -2991,9 +3014,8 @@ int emu_sv_dot(int n, int dtype, const void *a, const void *b, double *dots_out,
   return finalize(part, nb, 3, workspace, n, dots_out, stream);
 }
 
-int emu_sv_arnoldi_orthogonalize(int n, int dtype, int count, const void *const *vecs,
-                                 const double *scales, void *w, double *records, void *workspace,
-                                 hipStream_t stream) {
+int arnoldi_mgs(int n, int dtype, int count, const void *const *vecs, const double *scales, void *w,
+                double *records, void *workspace, int normalize, hipStream_t stream) {
   if (int rc = check_n(n)) return rc;
   if (int rc = check_dtype(dtype)) return rc;
   if (count < 1 || count > kMaxLincomb)
@@ -3006,8 +3028,8 @@ int emu_sv_arnoldi_orthogonalize(int n, int dtype, int count, const void *const 
   }
   if (n >= 10 && n <= kMgsSmallLog && count <= kMgsSmallCount)  // one workgroup, one launch
     return dtype == EMU_SV_C128
-               ? launch_mgs_small<double>(n, count, vecs, scales, w, records, stream)
-               : launch_mgs_small<float>(n, count, vecs, scales, w, records, stream);
+               ? launch_mgs_small<double>(n, count, vecs, scales, w, records, normalize, stream)
+               : launch_mgs_small<float>(n, count, vecs, scales, w, records, normalize, stream);
   const uint64_t npk = packets(n, dtype);
   const int nb = stream_blocks(npk);
   double *part = partials_ptr(workspace, n);
@@ -3025,7 +3047,159 @@ int emu_sv_arnoldi_orthogonalize(int n, int dtype, int count, const void *const 
     if (int rc = hip_check(hipGetLastError(), "arnoldi_mgs launch")) return rc;
     if (int rc = finalize(part, nb, 3, workspace, n, records + 4 * k, stream)) return rc;
   }
-  return EMU_SV_OK;
+  if (!normalize) return EMU_SV_OK;
+  if (dtype == EMU_SV_C128)
+    hipLaunchKernelGGL(scale_by_norm_kernel<double>, dim3(nb), dim3(kThreads), 0, stream, npk, w,
+                       records + 4 * count);
+  else
+    hipLaunchKernelGGL(scale_by_norm_kernel<float>, dim3(nb), dim3(kThreads), 0, stream, npk, w,
+                       records + 4 * count);
+  return hip_check(hipGetLastError(), "arnoldi normalisation launch");
+}
+
+int emu_sv_arnoldi_orthogonalize(int n, int dtype, int count, const void *const *vecs,
+                                 const double *scales, void *w, double *records, void *workspace,
+                                 hipStream_t stream) {
+  return arnoldi_mgs(n, dtype, count, vecs, scales, w, records, workspace, 0, stream);
+}
+
+int emu_sv_arnoldi_orthonormalize(int n, int dtype, int count, const void *const *vecs,
+                                  const double *scales, void *w, double *records, void *workspace,
+                                  hipStream_t stream) {
+  return arnoldi_mgs(n, dtype, count, vecs, scales, w, records, workspace, 1, stream);
+}
+
+// The whole Arnoldi exponential of one step on the library's host side: the
+// Arnoldi branch of krylov_exp (emu_base/math/krylov_exp.py:44-107 with
+// is_hermitian=False), pipelined.  Iteration j enqueues H.psi of the unit
+// vector v_j, the Gram-Schmidt of the result with its normalisation on the
+// device and an asynchronous copy of its records; iteration j+1 is enqueued
+// before the host waits for iteration j (unless half the last error estimate
+// is already below the tolerance), so the host's small exponential and test
+// overlap the device's next matvec.  No per-iteration interpreter work.
+namespace {
+struct EventPair {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  ~EventPair() {
+    for (hipEvent_t x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+};
+}  // namespace
+
+int emu_sv_arnoldi_exp(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
+                       void *v, void *out, void *basis, int max_dim, double exp_tolerance,
+                       double norm_tolerance, double *records, double *host_records,
+                       void *workspace, int tile_log, int min_contig, int flags,
+                       hipStream_t stream, int *result_out) {
+  if (int rc = check_n(n)) return rc;
+  if (int rc = check_dtype(dtype)) return rc;
+  if (!ham || !v || !out || !basis || !records || !host_records || !workspace || !result_out)
+    return fail(EMU_SV_EINVAL, "arnoldi_exp: null pointer");
+  if (max_dim < 1 || max_dim > kArnoldiMaxDim)
+    return fail(EMU_SV_EINVAL, "arnoldi_exp: max_dim must be in [1, 256]");
+  if (out == basis) return fail(EMU_SV_EINVAL, "arnoldi_exp: out must not alias the basis");
+  const size_t vbytes = (dtype == EMU_SV_C128 ? sizeof(double2) : sizeof(float2)) << n;
+  const size_t row = (size_t)(max_dim + 2) * 4;  // doubles per iteration's records
+  double *dot_dev = records + (size_t)max_dim * row;
+  double *dot_host = host_records + (size_t)max_dim * row;
+  const int hflags = flags & ~EMU_SV_FLAG_KEEP_DIAG;
+  EventPair ev;
+  for (hipEvent_t &x : ev.e)
+    if (int rc = hip_check(hipEventCreateWithFlags(&x, hipEventDisableTiming), "arnoldi_exp: event")) return rc;
+
+  // initial norm: the only synchronisation outside the iterations
+  if (int rc = emu_sv_dot(n, dtype, v, nullptr, dot_dev, workspace, stream)) return rc;
+  if (int rc = hip_check(hipMemcpyAsync(dot_host, dot_dev, 3 * sizeof(double), hipMemcpyDeviceToHost, stream),
+                         "arnoldi_exp: copy")) return rc;
+  if (int rc = hip_check(hipStreamSynchronize(stream), "arnoldi_exp: synchronize")) return rc;
+  const double initial_norm = std::sqrt(std::max(dot_host[0], 0.0));
+  if (!(initial_norm > 0.0)) return fail(EMU_SV_EINVAL, "arnoldi_exp: zero input vector");
+
+  std::vector<const void *> vec((size_t)max_dim + 1);
+  std::vector<double> sc((size_t)max_dim + 1, 1.0);  // later basis vectors are unit vectors
+  vec[0] = v;
+  sc[0] = 1.0 / initial_norm;
+  int enqueued = 0, matvecs = 0;
+  auto enqueue = [&](int j) -> int {
+    void *w = static_cast<char *>(basis) + (size_t)j * vbytes;
+    if (int rc = emu_sv_hpsi(n, dtype, ham, sigma_re, sigma_im, vec[j], sc[j], nullptr, 0.0, w, nullptr,
+                             workspace, tile_log, min_contig, hflags | (j > 0 ? EMU_SV_FLAG_KEEP_DIAG : 0),
+                             stream))
+      return rc;
+    double *rec = records + (size_t)j * row;
+    if (int rc = arnoldi_mgs(n, dtype, j + 1, vec.data(), sc.data(), w, rec, workspace, 1, stream)) return rc;
+    if (int rc = hip_check(hipMemcpyAsync(host_records + (size_t)j * row, rec, (size_t)(j + 2) * 4 * sizeof(double),
+                                          hipMemcpyDeviceToHost, stream), "arnoldi_exp: copy"))
+      return rc;
+    if (int rc = hip_check(hipEventRecord(ev.e[j & 1], stream), "arnoldi_exp: event")) return rc;
+    vec[j + 1] = w;
+    enqueued = j + 1;
+    ++matvecs;
+    return EMU_SV_OK;
+  };
+  using cd = std::complex<double>;
+  const int dim = max_dim + 2;
+  std::vector<cd> T((size_t)dim * dim, cd(0.0, 0.0));
+  std::vector<double> sub, col;
+  // col = expm(T[:m, :m]) e_0
+  auto expm_col = [&](int m) -> int {
+    sub.assign((size_t)2 * m * m, 0.0);
+    for (int r = 0; r < m; ++r)
+      for (int c = 0; c < m; ++c) {
+        sub[2 * ((size_t)r * m + c)] = T[(size_t)r * dim + c].real();
+        sub[2 * ((size_t)r * m + c) + 1] = T[(size_t)r * dim + c].imag();
+      }
+    col.assign((size_t)2 * m, 0.0);
+    if (emu_sv_expm_column(m, sub.data(), col.data()) != EMU_SV_OK)
+      return fail(EMU_SV_EINVAL, "arnoldi_exp: small exponential failed");
+    return EMU_SV_OK;
+  };
+  // out = initial_norm sum_k col_k sc_k V_k over the first count vectors
+  auto combine = [&](int count) -> int {
+    std::vector<double> coef((size_t)2 * count);
+    for (int k = 0; k < count; ++k) {
+      coef[2 * k] = col[2 * k] * sc[k] * initial_norm;
+      coef[2 * k + 1] = col[2 * k + 1] * sc[k] * initial_norm;
+    }
+    return emu_sv_lincomb(n, dtype, count, vec.data(), coef.data(), out, stream);
+  };
+  auto finish = [&](int count, int iterations, int converged, int happy) -> int {
+    if (int rc = combine(count)) return rc;
+    result_out[0] = iterations;
+    result_out[1] = converged;
+    result_out[2] = happy;
+    result_out[3] = matvecs;
+    return EMU_SV_OK;
+  };
+
+  if (int rc = enqueue(0)) return rc;
+  double last_err = -1.0;
+  for (int j = 0; j < max_dim; ++j) {
+    const bool likely_done = last_err >= 0.0 && 0.5 * last_err < exp_tolerance;
+    if (j + 1 < max_dim && !likely_done)
+      if (int rc = enqueue(j + 1)) return rc;
+    if (int rc = hip_check(hipEventSynchronize(ev.e[j & 1]), "arnoldi_exp: wait")) return rc;
+    const double *rec = host_records + (size_t)j * row;
+    const double nrm = std::sqrt(std::max(rec[0], 0.0));  // ||op(v_j)||, before orthogonalisation
+    for (int k = 0; k <= j; ++k) T[(size_t)k * dim + j] = sc[k] * cd(rec[4 * k + 1], rec[4 * k + 2]);
+    const double n2 = std::sqrt(std::max(rec[4 * (j + 1)], 0.0));
+    T[(size_t)(j + 1) * dim + j] = n2;
+    if (n2 < norm_tolerance) {  // happy breakdown
+      if (int rc = expm_col(j + 1)) return rc;
+      return finish(j + 1, j + 1, 1, 1);
+    }
+    T[(size_t)(j + 2) * dim + j + 1] = 1.0;
+    if (int rc = expm_col(j + 3)) return rc;
+    const double err1 = std::abs(cd(col[2 * (j + 1)], col[2 * (j + 1) + 1]));
+    const double err2 = std::abs(cd(col[2 * (j + 2)], col[2 * (j + 2) + 1])) * nrm;
+    const double err = err1 < err2 ? err1 : err1 * err2 / (err1 - err2);
+    if (err < exp_tolerance) return finish(j + 2, j + 1, 1, 0);
+    last_err = err;
+    if (j + 1 < max_dim && enqueued == j + 1)
+      if (int rc = enqueue(j + 1)) return rc;  // the speculation was skipped and j did not converge
+  }
+  return finish(max_dim + 1, max_dim, 0, 0);
 }
 
 int emu_sv_apply_1q(int n, int dtype, void *psi, int bit, const double *a, hipStream_t stream) {

@@ -405,6 +405,137 @@ def fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
     return _fused_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim, rows)
 
 
+def _arnoldi_matvec(op, v: torch.Tensor, n: int):
+    """dst = op(scale * src) on flat device vectors for the Arnoldi engines."""
+    if not isinstance(op, ScaledHamiltonian):
+        return op.device_matvec
+    ham = op.hamiltonian
+    if n != ham.nqubits:
+        raise ValueError(f"vector has {n} qubits, the Hamiltonian {ham.nqubits}")
+    params = ham.device_params(v.device)
+    sched = ham._schedule(v)
+    built = []
+
+    def matvec(src, scale, dst):
+        # the first call of this run builds the diagonal table; the later
+        # ones keep it (the library rebuilds it if another block intervened)
+        native.hpsi(n, params, op.sigma, src, scale, dst, complex_coef=ham.complex,
+                    noise=ham.noisy, keep_diag=bool(built), **sched.kwargs())
+        built.append(True)
+
+    return matvec
+
+
+class _ArnoldiBuffers:
+    """Per-(device, stream) pinned host rows of the pipelined Arnoldi engine:
+    iteration j's (j+2) x 4 Gram-Schmidt records land in host[j] as they
+    complete, with events[j] recorded after the copy."""
+
+    _cache: dict = {}
+
+    def __init__(self, rows: int):
+        self.host = torch.zeros((rows, rows + 2, 4), dtype=torch.float64, pin_memory=True)
+        self.events = [torch.cuda.Event() for _ in range(rows)]
+        self.rows = rows
+
+    @classmethod
+    def get(cls, device: torch.device, rows: int) -> "_ArnoldiBuffers":
+        key = native.stream_key(device)
+        b = cls._cache.get(key)
+        if b is None or b.rows < rows:
+            b = cls._cache[key] = cls(rows)
+        return b
+
+
+def _pipelined_arnoldi(op, v: torch.Tensor, exp_tolerance: float, norm_tolerance: float,
+                       max_krylov_dim: int) -> KrylovExpResult:
+    """The Arnoldi branch with the host one iteration behind the device.
+    Each new basis vector is normalised on the device by its own Gram-Schmidt
+    launch (emu_sv_arnoldi_orthonormalize: v_{j+1} = w / T[j+1, j], the
+    reference's ``w /= n2``), so iteration j+1 is enqueued before the host
+    has read iteration j's records; the host then builds column j of T,
+    evaluates the Expokit test and decides.  Iteration j+1 is not enqueued
+    ahead when iteration j is likely to converge (half the last error
+    estimate below the tolerance, as in the Lanczos pipeline); a discarded
+    speculative iteration costs one matvec.  Same arithmetic as
+    ``_fused_arnoldi`` apart from the rounding of the stored unit vectors."""
+    shape = v.shape
+    v = v.contiguous().view(-1)
+    n = native.nqubits_of(v)
+    matvec = _arnoldi_matvec(op, v, n)
+    initial_norm = math.sqrt(float(native.dot(n, v)[0].item()))
+    basis = [v]
+    scales = [1.0 / initial_norm]  # later basis vectors are unit vectors
+    T = np.zeros((max_krylov_dim + 2, max_krylov_dim + 2), dtype=np.complex128)
+    bufs = _ArnoldiBuffers.get(v.device, max_krylov_dim)
+
+    def enqueue(j: int) -> None:
+        w = torch.empty_like(v)
+        matvec(basis[j], scales[j], w)
+        KrylovStats.matvecs += 1
+        rec = native.arnoldi_orthogonalize(n, basis[: j + 1], scales[: j + 1], w, normalize=True)
+        bufs.host[j, : j + 2].copy_(rec, non_blocking=True)
+        bufs.events[j].record()
+        basis.append(w)
+        scales.append(1.0)
+
+    def combine(coeffs) -> torch.Tensor:
+        c = np.asarray(coeffs, dtype=np.complex128) * np.asarray(scales[: len(coeffs)]) * initial_norm
+        return native.lincomb(n, basis[: len(coeffs)], c, torch.empty_like(v)).view(shape)
+
+    enqueue(0)
+    errs: list = []
+    expd = None
+    for j in range(max_krylov_dim):
+        likely_done = len(errs) >= 1 and 0.5 * errs[-1] < exp_tolerance
+        if j + 1 < max_krylov_dim and not likely_done:
+            enqueue(j + 1)
+        bufs.events[j].synchronize()
+        rec = bufs.host[j, : j + 2].numpy()
+        nrm = math.sqrt(max(rec[0, 0], 0.0))  # ||op(v_j)||, before orthogonalisation
+        for k in range(j + 1):
+            T[k, j] = scales[k] * complex(rec[k, 1], rec[k, 2])
+        n2 = math.sqrt(max(rec[j + 1, 0], 0.0))
+        T[j + 1, j] = n2
+        if n2 < norm_tolerance:  # happy breakdown
+            expd = native.expm_column(T[: j + 1, : j + 1])[:, None]
+            return KrylovExpResult(combine(expd[:, 0]), True, True, j + 1)
+        T[j + 2, j + 1] = 1.0
+        expd = native.expm_column(T[: j + 3, : j + 3])[:, None]
+        err = _error_estimate(expd, j, nrm)
+        if err < exp_tolerance:
+            return KrylovExpResult(combine(expd[: j + 2, 0]), True, False, j + 1)
+        errs.append(err)
+        if j + 1 < max_krylov_dim and len(basis) == j + 2:
+            enqueue(j + 1)  # the speculation was skipped and j did not converge
+    return KrylovExpResult(combine(expd[: max_krylov_dim + 1, 0]), False, False, max_krylov_dim)
+
+
+# Up to this many qubits the Arnoldi iteration runs in the library's driver
+# (emu_sv_arnoldi_exp; its basis buffer holds max_krylov_dim vectors): there
+# the per-iteration work is a few tens of microseconds of GPU time, which
+# interpreter work per iteration would otherwise exceed.
+NATIVE_ARNOLDI_MAX_QUBITS = 16
+
+
+def _native_arnoldi(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
+                    norm_tolerance: float, max_krylov_dim: int) -> KrylovExpResult:
+    """The Arnoldi branch in one library call (emu_sv_arnoldi_exp: the loop
+    of _pipelined_arnoldi in native code)."""
+    shape = v.shape
+    v = v.contiguous().view(-1)
+    n = native.nqubits_of(v)
+    ham = op.hamiltonian
+    if n != ham.nqubits:
+        raise ValueError(f"vector has {n} qubits, the Hamiltonian {ham.nqubits}")
+    out = torch.empty_like(v)
+    iters, converged, happy, matvecs = native.arnoldi_exp(
+        n, ham.device_params(v.device), op.sigma, v, out, max_krylov_dim, exp_tolerance,
+        norm_tolerance, complex_coef=ham.complex, noise=ham.noisy, **ham._schedule(v).kwargs())
+    KrylovStats.matvecs += matvecs
+    return KrylovExpResult(out.view(shape), converged, happy, iters)
+
+
 def _fused_arnoldi(op, v: torch.Tensor, exp_tolerance: float,
                    norm_tolerance: float, max_krylov_dim: int) -> KrylovExpResult:
     """Arnoldi branch of krylov_exp_impl (krylov_exp.py:44-107 with
@@ -412,27 +543,17 @@ def _fused_arnoldi(op, v: torch.Tensor, exp_tolerance: float,
     ``ScaledHamiltonian`` or an operator with ``device_matvec(src, scale,
     dst)`` (dst = op(scale * src) on flat device vectors, e.g. the scaled
     Lindbladian on a density matrix).  Basis vectors are stored unnormalised
-    with a host-side scale; v itself is basis vector 0."""
+    with a host-side scale; v itself is basis vector 0.  One host round trip
+    per iteration (EMU_SV_PIPELINE=0; the default is _pipelined_arnoldi)."""
+    if PIPELINE:
+        if (isinstance(op, ScaledHamiltonian) and native.nqubits_of(v) <= NATIVE_ARNOLDI_MAX_QUBITS
+                and max_krylov_dim <= 256):
+            return _native_arnoldi(op, v, exp_tolerance, norm_tolerance, max_krylov_dim)
+        return _pipelined_arnoldi(op, v, exp_tolerance, norm_tolerance, max_krylov_dim)
     shape = v.shape
     v = v.contiguous().view(-1)
     n = native.nqubits_of(v)
-    if isinstance(op, ScaledHamiltonian):
-        ham = op.hamiltonian
-        if n != ham.nqubits:
-            raise ValueError(f"vector has {n} qubits, the Hamiltonian {ham.nqubits}")
-        params = ham.device_params(v.device)
-        sched = ham._schedule(v)
-
-        built = []
-
-        def matvec(src, scale, dst):
-            # the first call of this run builds the diagonal table; the later
-            # ones keep it (the library rebuilds it if another block intervened)
-            native.hpsi(n, params, op.sigma, src, scale, dst, complex_coef=ham.complex,
-                        noise=ham.noisy, keep_diag=bool(built), **sched.kwargs())
-            built.append(True)
-    else:
-        matvec = op.device_matvec
+    matvec = _arnoldi_matvec(op, v, n)
     initial_norm = math.sqrt(float(native.dot(n, v)[0].item()))
     basis = [v]
     scales = [1.0 / initial_norm]

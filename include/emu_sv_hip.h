@@ -273,6 +273,40 @@ int emu_sv_arnoldi_orthogonalize(int n, int dtype, int count, const void *const 
                                  const double *scales, void *w, double *records_out,
                                  void *workspace, hipStream_t stream);
 
+/* emu_sv_arnoldi_orthogonalize, then w <- w / ||w|| with the norm after the
+ * last subtraction (records_out[4*count], still the unnormalised value), on
+ * the device: w becomes the next orthonormal Arnoldi vector v_{j+1} =
+ * w / T[j+1, j] (krylov_exp.py:60-61,74) without a host round trip, so the
+ * next iteration can be enqueued before the host has read this one.  A zero
+ * norm (exact breakdown) leaves w as it is. */
+int emu_sv_arnoldi_orthonormalize(int n, int dtype, int count, const void *const *vecs,
+                                  const double *scales, void *w, double *records_out,
+                                  void *workspace, hipStream_t stream);
+
+/* out = exp(sigma H) v by the Arnoldi branch of krylov_exp
+ * (emu_base/math/krylov_exp.py:23-107 with is_hermitian=False) with its loop
+ * inside the library: per iteration emu_sv_hpsi (flags: kernel kind,
+ * EMU_SV_FLAG_COMPLEX, EMU_SV_FLAG_NOISE; KEEP_DIAG is set by the driver
+ * after the first matvec), emu_sv_arnoldi_orthonormalize and an
+ * asynchronous copy of the records; the host's column exponential
+ * (emu_sv_expm_column), the Expokit estimate, the happy-breakdown test and
+ * the final combination (emu_sv_lincomb) follow the reference.  Iteration
+ * j+1 is enqueued before the host waits for iteration j, unless half the
+ * last error estimate is below the tolerance; a discarded speculative
+ * iteration costs one matvec.  Synchronises the stream.
+ *   v          device input state (2^n elements, not modified; out may alias it)
+ *   basis      device buffer of max_dim vectors of 2^n elements (v_1 .. v_max_dim)
+ *   records    device, (max_dim * (max_dim + 2) + 1) * 4 doubles
+ *   host_records  pinned host buffer of the same size
+ *   result_out host int[4]: iterations, converged, happy breakdown, matvecs
+ * max_dim in [1, 256].  Returns EMU_SV_OK also when the iteration did not
+ * converge (result_out[1] == 0: the reference's RecursionError). */
+int emu_sv_arnoldi_exp(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
+                       void *v, void *out, void *basis, int max_dim, double exp_tolerance,
+                       double norm_tolerance, double *records, double *host_records,
+                       void *workspace, int tile_log, int min_contig, int flags,
+                       hipStream_t stream, int *result_out);
+
 /* psi <- (A acting on qubit bit b) psi, A a 2x2 complex matrix (8 doubles
  * row-major (re, im), host) in the (bit 0, bit 1) = (g, r) basis: for each
  * pair (x, x | 2^b) with bit b clear, (a0, a1) <- (A00 a0 + A01 a1,

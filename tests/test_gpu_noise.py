@@ -107,6 +107,99 @@ def test_arnoldi_records_vs_mgs(gpu_device, n, count, dtype):
     assert float((out - ww).abs().max()) <= tol * max(1.0, float(w.abs().max()))
 
 
+@pytest.mark.parametrize("n,count", [(4, 1), (8, 5), (10, 12), (13, 9), (15, 4)])
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_arnoldi_orthonormalize(gpu_device, n, count, dtype):
+    """emu_sv_arnoldi_orthonormalize: the records of emu_sv_arnoldi_orthogonalize
+    (norm after the last subtraction unnormalised) and w divided by that norm
+    on the device (one-workgroup path for 10 <= n <= 13, launch chain
+    otherwise); a zero w stays zero."""
+    from emulators_amd import _native as native
+    print(f"arnoldi orthonormalize n={n} count={count} {dtype}", flush=True)
+    torch.manual_seed(300 + n)
+    basis = [torch.randn(2**n, dtype=torch.complex128).to(gpu_device, dtype) for _ in range(count)]
+    scales = [1.0 / float(b.norm()) for b in basis]
+    w = torch.randn(2**n, dtype=torch.complex128).to(gpu_device, dtype)
+    w1, w2 = w.clone(), w.clone()
+    rec1 = native.arnoldi_orthogonalize(n, basis, scales, w1).cpu().numpy()
+    rec2 = native.arnoldi_orthogonalize(n, basis, scales, w2, normalize=True).cpu().numpy()
+    assert np.array_equal(rec1[:, :3], rec2[:, :3])
+    unit = w1.to(torch.complex128) / math.sqrt(rec1[count, 0])
+    tol = 1e-14 if dtype == torch.complex128 else 2e-7
+    assert float((w2.to(torch.complex128) - unit).abs().max()) <= tol
+    z = torch.zeros_like(w)
+    native.arnoldi_orthogonalize(n, basis[:1], scales[:1], z, normalize=True)
+    assert float(z.abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("n,dt,tol", [(8, 0.5, 1e-10), (10, 1.0, 1e-8), (14, 0.3, 1e-10)])
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_arnoldi_engines_agree(gpu_device, monkeypatch, n, dt, tol, dtype):
+    """The three Arnoldi engines on one noisy step: the one-read-back engine
+    (unnormalised basis, host scales), the pipelined one (device-normalised
+    basis, host one iteration behind) and the library's driver
+    (emu_sv_arnoldi_exp, the same loop in native code).  The same iteration
+    count and the same result to rounding: 1e-12 in complex128; in
+    complex64 the engines round different basis vectors (unit against
+    unnormalised) at 6e-8 each over about 20 iterations, so 1e-5 relative.
+    The pipelined and the native engine store the same vectors: 1e-12 /
+    1e-6 between them."""
+    import importlib
+
+    from emulators_amd.emu_sv.hamiltonian import RydbergHamiltonian
+    kx = importlib.import_module("emulators_amd.emu_base.math.krylov_exp")
+    print(f"arnoldi engines n={n} dt={dt} {dtype}", flush=True)
+    om, de, ph, u = _params(n, 60 + n, True)
+    h = RydbergHamiltonian(om, de, ph, u, device=gpu_device, noise=orc.noise_from_lindbladians(LINDBLADS))
+    torch.manual_seed(n)
+    psi = torch.randn(2**n, dtype=torch.complex128).to(gpu_device, dtype)
+    op = kx.ScaledHamiltonian(h, -1j * dt)
+    monkeypatch.setattr(kx, "PIPELINE", False)
+    plain = kx.krylov_exp_impl(op, psi.clone(), is_hermitian=False, exp_tolerance=tol, norm_tolerance=tol)
+    piped = kx._pipelined_arnoldi(op, psi.clone(), tol, tol, 100)
+    native = kx._native_arnoldi(op, psi.clone(), tol, tol, 100)
+    scale = float(plain.result.abs().max())
+    for got, bound in ((piped, 1e-12 if dtype == torch.complex128 else 1e-5),
+                       (native, 1e-12 if dtype == torch.complex128 else 1e-5)):
+        assert got.converged and not got.happy_breakdown
+        assert got.iteration_count == plain.iteration_count
+        diff = float((got.result - plain.result).abs().max())
+        assert diff <= bound * scale, diff
+    diff = float((native.result - piped.result).abs().max())
+    assert diff <= (1e-12 if dtype == torch.complex128 else 1e-6) * scale, diff
+    # the dispatcher takes the native driver for n <= 16
+    monkeypatch.setattr(kx, "PIPELINE", True)
+    calls = []
+    real = kx._native_arnoldi
+    monkeypatch.setattr(kx, "_native_arnoldi", lambda *a, **k: calls.append(1) or real(*a, **k))
+    kx.krylov_exp_impl(op, psi.clone(), is_hermitian=False, exp_tolerance=tol, norm_tolerance=tol)
+    assert calls
+
+
+def test_native_arnoldi_happy_breakdown(gpu_device):
+    """A basis state under a diagonal noisy H (no drive, dephasing only) is an
+    eigenvector: the first Gram-Schmidt leaves nothing, the driver reports a
+    happy breakdown after one iteration and returns exp(dt E) psi."""
+    import importlib
+
+    from emulators_amd.emu_sv.hamiltonian import RydbergHamiltonian
+    kx = importlib.import_module("emulators_amd.emu_base.math.krylov_exp")
+    n = 6
+    _, de, _, u = _params(n, 3, False)
+    deph = [math.sqrt(0.4) * torch.tensor([[1, 0], [0, -1]], dtype=torch.complex128)]
+    k = orc.noise_from_lindbladians(deph)
+    h = RydbergHamiltonian(torch.zeros(n, dtype=torch.float64), de, torch.zeros(n, dtype=torch.float64), u,
+                           device=gpu_device, noise=k)
+    psi = torch.zeros(2**n, dtype=torch.complex128, device=gpu_device)
+    psi[5] = 0.5
+    dt = 0.3
+    got = kx._native_arnoldi(kx.ScaledHamiltonian(h, -1j * dt), psi.clone(), 1e-10, 1e-10, 100)
+    assert got.converged and got.happy_breakdown and got.iteration_count == 1
+    dense = orc.dense_noisy_hamiltonian(torch.zeros(n), de, torch.zeros(n), u, k)
+    expected = torch.linalg.matrix_exp(-1j * dt * dense) @ psi.cpu()
+    assert float((got.result.cpu() - expected).abs().max()) < 1e-12
+
+
 @pytest.mark.parametrize("n,dt,tol", [(4, 0.3, 1e-10), (8, 0.5, 1e-10), (10, 1.0, 1e-8)])
 @pytest.mark.parametrize("phase", [False, True])
 def test_arnoldi_krylov_vs_oracle(gpu_device, n, dt, tol, phase):
