@@ -351,3 +351,18 @@ def test_iterate_dev_equals_matvec_dev_then_next_coef(gpu_device, kernel, sched,
     for a, b in zip(*outs):
         assert torch.equal(a, b)
     assert float(outs[1][2][0]) > 0.0 and torch.all(torch.isfinite(outs[1][3]))
+
+
+def test_peak_memory_probe_matches_torch(gpu_device):
+    """The per-step statistics' GPU peak-memory probe reads the allocator's
+    peak counter, the value torch.cuda.max_memory_allocated reports
+    (emu_base/utils.py:94-99)."""
+    from emulators_amd.emu_base import get_max_rss_gpu
+    from emulators_amd.emu_base.utils import _peak_allocated
+
+    x = torch.empty(3 << 20, dtype=torch.float64, device=gpu_device)
+    for d in range(torch.cuda.device_count()):
+        assert _peak_allocated(d) == torch.cuda.max_memory_allocated(d)
+    peaks = [torch.cuda.max_memory_allocated(d) for d in range(torch.cuda.device_count())]
+    assert get_max_rss_gpu() == max(peaks) * 1e-6
+    del x
