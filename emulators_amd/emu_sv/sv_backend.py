@@ -23,6 +23,7 @@ class SVBackend:
         assert isinstance(self._config, SVConfig)
         if isinstance(self._sequence, SequenceData):
             return self._run_from_sequence_data(self._sequence, self._config)
+        assert getattr(self._sequence, "_in_ising", True), "Only Rydberg interaction is currently supported"
         pulser_data = PulserData(sequence=self._sequence, config=self._config, dt=self._config.dt)
         return Results.aggregate([self._run_from_sequence_data(sd, self._config)
                                   for sd in pulser_data.get_sequences()])
