@@ -1000,6 +1000,10 @@ struct MgsSmallArgs {
   double s2[kMgsSmallCount];
 };
 
+// Workgroup sums with one barrier: consecutive calls alternate between two
+// buffers (red = the call's own one).  A thread writing buffer b in call k+2
+// has passed call k+1's barrier, which every thread reaches only after its
+// reads of buffer b in call k, so no second barrier is needed.
 template <int NT, int K>
 __device__ __forceinline__ void mgs_block_sum(double (&v)[K], double *red) {
   constexpr int NW = NT / 64;
@@ -1017,7 +1021,6 @@ __device__ __forceinline__ void mgs_block_sum(double (&v)[K], double *red) {
     for (int w = 0; w < NW; ++w) sm += red[w * K + k];  // same order in every thread
     v[k] = sm;
   }
-  __syncthreads();
 }
 
 template <typename T, int EPT, int NT>
@@ -1025,7 +1028,7 @@ __global__ __launch_bounds__(NT) void arnoldi_mgs_small_kernel(int count, void *
                                                                const MgsSmallArgs args,
                                                                double *records, int normalize) {
   using T2 = typename Vec2<T>::type;
-  __shared__ double red[(NT / 64) * 3];
+  __shared__ double red[2][(NT / 64) * 3];  // alternating per workgroup sum
   T2 *w = reinterpret_cast<T2 *>(wv);
   double wr[EPT], wi[EPT];
 #pragma unroll
@@ -1050,7 +1053,7 @@ __global__ __launch_bounds__(NT) void arnoldi_mgs_small_kernel(int count, void *
       acc[1] += ur[e] * wr[e] + ui[e] * wi[e];
       acc[2] += ur[e] * wi[e] - ui[e] * wr[e];
     }
-    mgs_block_sum<NT, 3>(acc, red);
+    mgs_block_sum<NT, 3>(acc, red[k & 1]);
     if (threadIdx.x == 0) {
       records[4 * k] = acc[0];
       records[4 * k + 1] = acc[1];
@@ -1068,7 +1071,7 @@ __global__ __launch_bounds__(NT) void arnoldi_mgs_small_kernel(int count, void *
   double nrm[1] = {0.0};
 #pragma unroll
   for (int e = 0; e < EPT; ++e) nrm[0] += wr[e] * wr[e] + wi[e] * wi[e];
-  mgs_block_sum<NT, 1>(nrm, red);
+  mgs_block_sum<NT, 1>(nrm, red[count & 1]);
   if (threadIdx.x == 0) records[4 * count] = nrm[0];
   // normalize: store the unit vector (the next basis vector), rounded once
   const double sc = (normalize && nrm[0] > 0.0) ? 1.0 / sqrt(nrm[0]) : 1.0;
@@ -1851,7 +1854,14 @@ struct OrthoIn {
 // leaves rows whose stamp matches and rebuilds the others, so a table is
 // never reused for a different block, size or tile, whatever address the
 // block has.  Without keep every row is rebuilt.
+// Set by emu_sv_arnoldi_exp around its matvecs after the first: the table
+// was built by the first one, on this workspace and stream, with nothing in
+// between, so the stamp check (a launch on the iteration's critical path)
+// is skipped.  Calling-thread state, never set across a return to the caller.
+thread_local bool t_diag_known_fresh = false;
+
 int prepare_diag(int n, int t1, const double *ham, void *ws, bool keep, hipStream_t st) {
+  if (keep && t_diag_known_fresh) return EMU_SV_OK;
   const uint64_t ntiles = 1ull << (n - t1);
   hipLaunchKernelGGL(diag_prep_kernel, dim3((unsigned)((ntiles * kDiagRow + kThreads - 1) / kThreads)),
                      dim3(kThreads), 0, st, n, t1, ham, const_cast<double *>(dtab_ptr(ws, n)),
@@ -3123,10 +3133,12 @@ int emu_sv_arnoldi_exp(int n, int dtype, const double *ham, double sigma_re, dou
   int enqueued = 0, matvecs = 0;
   auto enqueue = [&](int j) -> int {
     void *w = static_cast<char *>(basis) + (size_t)j * vbytes;
-    if (int rc = emu_sv_hpsi(n, dtype, ham, sigma_re, sigma_im, vec[j], sc[j], nullptr, 0.0, w, nullptr,
-                             workspace, tile_log, min_contig, hflags | (j > 0 ? EMU_SV_FLAG_KEEP_DIAG : 0),
-                             stream))
-      return rc;
+    t_diag_known_fresh = j > 0;
+    const int hrc = hpsi_impl(n, dtype, ham, sigma_re, sigma_im, vec[j], sc[j], nullptr, 0.0, w, nullptr,
+                              workspace, tile_log, min_contig, hflags | (j > 0 ? EMU_SV_FLAG_KEEP_DIAG : 0),
+                              stream);
+    t_diag_known_fresh = false;
+    if (hrc) return hrc;
     double *rec = records + (size_t)j * row;
     if (int rc = arnoldi_mgs(n, dtype, j + 1, vec.data(), sc.data(), w, rec, workspace, 1, stream)) return rc;
     if (int rc = hip_check(hipMemcpyAsync(host_records + (size_t)j * row, rec, (size_t)(j + 2) * 4 * sizeof(double),
