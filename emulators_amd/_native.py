@@ -286,7 +286,7 @@ def hpsi(n: int, ham_dev: torch.Tensor, sigma: complex, v: torch.Tensor, v_scale
          min_contig: int = 0, kernel: str = "default", partners: int = 0,
          first_log: int = 0, noise: bool = False, keep_diag: bool = False) -> torch.Tensor | None:
     """w = sigma * H (v_scale v); optionally returns the 7 fused dots (device).
-    ``noise``: the block carries the non-Hermitian term (LDS kernel).
+    ``noise``: the block carries the non-Hermitian term (LDS or btile kernel).
     ``keep_diag``: reuse the workspace's diagonal table when it was built for
     this block, n and first-pass tile (EMU_SV_FLAG_KEEP_DIAG)."""
     _require_device_vec(v, "hpsi")

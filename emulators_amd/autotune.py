@@ -146,6 +146,63 @@ def best_schedule(n: int, dtype: torch.dtype, device: torch.device, params: torc
     return sched
 
 
+# Noisy H.psi (the Monte-Carlo effective Hamiltonian, EMU_SV_FLAG_NOISE): the
+# LDS and big-tile kernels carry the noise term.  The Arnoldi iteration calls
+# bare H.psi (no orthogonalisation), so the candidates are timed on that.
+NOISE_CANDIDATES = {
+    torch.complex64: [Schedule("lds", 0, 0), Schedule("lds", 11, 3), Schedule("btile", 0, 4),
+                      Schedule("btile", 0, 5)],
+    torch.complex128: [Schedule("lds", 0, 0), Schedule("lds", 11, 2), Schedule("btile", 0, 2),
+                       Schedule("btile", 0, 4)],
+}
+NOISE_DEFAULT = Schedule("lds", 0, 0)
+_noise_cache: dict = {}
+
+
+def time_noise_schedule(n: int, params: torch.Tensor, sched: Schedule, vecs, warmup: int = 2,
+                        reps: int = 3) -> float:
+    """Average ms of one noisy bare H.psi (emu_sv_hpsi with EMU_SV_FLAG_NOISE)."""
+    v, w = vecs
+
+    def call():
+        native.hpsi(n, params, -0.01j, v, 1.0, w, complex_coef=True, noise=True, **sched.kwargs())
+
+    for _ in range(warmup):
+        call()
+    stream = torch.cuda.current_stream(v.device)
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(reps):
+        call()
+    stop.record(stream)
+    stop.synchronize()
+    return start.elapsed_time(stop) / reps
+
+
+def best_noise_schedule(n: int, dtype: torch.dtype, device: torch.device,
+                        params: torch.Tensor) -> Schedule:
+    """The fastest noisy H.psi schedule for this shape (NOISE_DEFAULT below
+    MIN_TUNED_QUBITS or when tuning is disabled; EMU_SV_SCHEDULE applies when
+    it names a kernel that carries the noise term)."""
+    forced = forced_schedule(dtype)
+    if forced is not None and forced.kernel in ("lds", "btile"):
+        return forced
+    if n < MIN_TUNED_QUBITS or not _enabled():
+        return NOISE_DEFAULT
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    key = (idx, n, dtype, "noise")
+    sched = _noise_cache.get(key)
+    if sched is None:
+        g = torch.Generator(device=device).manual_seed(1234)
+        vecs = tuple(torch.randn(2**n, dtype=dtype, device=device, generator=g) for _ in range(2))
+        times = {c: time_noise_schedule(n, params, c, vecs) for c in NOISE_CANDIDATES[dtype]}
+        del vecs
+        sched = _noise_cache[key] = min(times, key=times.get)
+        _timings[key] = times
+    return sched
+
+
 # Batch engine: (tile_log, min_contig) candidates of the LDS pass kernel per
 # element type, and the tuned choice per (device, dtype, atom count).
 BATCH_CANDIDATES = {
@@ -236,5 +293,5 @@ def batch_tuned_table() -> dict:
 
 
 def tuned_table() -> dict:
-    """{(device, n, dtype, complex): {schedule: ms}} of every tuning done so far."""
+    """{(device, n, dtype, complex or "noise"): {schedule: ms}} of every tuning done so far."""
     return dict(_timings)

@@ -2146,36 +2146,45 @@ int check_btile_schedule(int n, int dtype, int &cmin, const char *what) {
   return n <= tb ? 1 : 0;
 }
 
-template <typename T, bool CPLX, bool MODEB, bool DOTS>
+template <typename T, bool CPLX, bool MODEB, bool DOTS, bool NOISE>
 int launch_btile_one(const BtArgs &a, hipStream_t st) {
   static std::once_flag once;
   static hipError_t attr = hipSuccess;
   const size_t lds = (size_t)kBtRoundSlots * kBtThreads * 16;
   std::call_once(once, [&] {
-    attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&btile_kernel<T, CPLX, MODEB, DOTS>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&btile_kernel<T, CPLX, MODEB, DOTS, NOISE>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   });
   if (int rc = hip_check(attr, "hipFuncSetAttribute(btile kernel LDS)")) return rc;
   const uint64_t nt = 1ull << (a.n - bt_tile_bits<T>());
-  hipLaunchKernelGGL((btile_kernel<T, CPLX, MODEB, DOTS>), dim3((unsigned)nt), dim3(kBtThreads), lds,
-                     st, a);
+  hipLaunchKernelGGL((btile_kernel<T, CPLX, MODEB, DOTS, NOISE>), dim3((unsigned)nt),
+                     dim3(kBtThreads), lds, st, a);
   return hip_check(hipGetLastError(), "btile kernel launch");
 }
 
-template <typename T, bool CPLX>
+template <typename T, bool CPLX, bool NOISE>
 int launch_btile_c(const BtArgs &a, bool modeb, bool dots, hipStream_t st) {
-  if (!modeb) return launch_btile_one<T, CPLX, false, false>(a, st);
-  return dots ? launch_btile_one<T, CPLX, true, true>(a, st) : launch_btile_one<T, CPLX, true, false>(a, st);
+  if (!modeb) return launch_btile_one<T, CPLX, false, false, NOISE>(a, st);
+  return dots ? launch_btile_one<T, CPLX, true, true, NOISE>(a, st)
+              : launch_btile_one<T, CPLX, true, false, NOISE>(a, st);
 }
 
 template <typename T>
-int launch_btile(const BtArgs &a, bool cplx, bool modeb, bool dots, hipStream_t st) {
-  return cplx ? launch_btile_c<T, true>(a, modeb, dots, st) : launch_btile_c<T, false>(a, modeb, dots, st);
+int launch_btile(const BtArgs &a, bool cplx, bool noise, bool modeb, bool dots, hipStream_t st) {
+  if (noise) return launch_btile_c<T, true, true>(a, modeb, dots, st);
+  return cplx ? launch_btile_c<T, true, false>(a, modeb, dots, st)
+              : launch_btile_c<T, false, false>(a, modeb, dots, st);
 }
 
+// noise: the Monte-Carlo term K of the parameter block (bare H.psi only:
+// no orthogonalisation, which needs a Hermitian H)
 template <typename T>
 int run_btile(int n, const double *ham, double sr, double si, const void *v, double vsc, void *w,
-              double *dots, void *ws, int cmin, bool cplx, hipStream_t st, const OrthoIn *ortho) {
+              double *dots, void *ws, int cmin, bool cplx, hipStream_t st, const OrthoIn *ortho,
+              bool noise = false) {
+  if (noise && ortho)
+    return fail(EMU_SV_EINVAL, "btile: the noise term is for bare H.psi (no orthogonalisation)");
   constexpr int TB = bt_tile_bits<T>();
   Pass plan[EMU_SV_MAX_QUBITS];
   const int np = make_btile_plan(n, TB, cmin, plan);
@@ -2210,7 +2219,7 @@ int run_btile(int n, const double *ham, double sr, double si, const void *v, dou
     const bool want_dots = last && dots != nullptr;
     if (want_dots && p == 0)
       return fail(EMU_SV_EINVAL, "btile: the reductions need a second pass (n above one tile)");
-    int rc = launch_btile<T>(a, cplx, p > 0, want_dots, st);
+    int rc = launch_btile<T>(a, cplx || noise, noise, p > 0, want_dots, st);
     if (rc) return rc;
     const int nrow = (int)(1ull << (n - TB));
     if (a.ortho) {
@@ -2773,22 +2782,22 @@ int hpsi_impl(int n, int dtype, const double *ham, double sigma_re, double sigma
   const bool noise = (flags & EMU_SV_FLAG_NOISE) != 0;
   const bool cplx = noise || (flags & EMU_SV_FLAG_COMPLEX) != 0;
   const bool keep = (flags & EMU_SV_FLAG_KEEP_DIAG) != 0;
-  if (kind == kKernelBtile && !noise) {
+  if (kind == kKernelBtile) {
     const int rc = check_btile_schedule(n, dtype, min_contig, "hpsi");
     if (rc < 0) return rc;
     if (rc == 0 && !prev) {  // the btile passes do not reduce the prev dots
       if (!workspace) return fail(EMU_SV_EINVAL, "hpsi: null workspace");
       if (dtype == EMU_SV_C128)
         return run_btile<double>(n, ham, sigma_re, sigma_im, v, v_scale, w, dots_out, workspace,
-                                 min_contig, cplx, stream, nullptr);
+                                 min_contig, cplx, stream, nullptr, noise);
       return run_btile<float>(n, ham, sigma_re, sigma_im, v, v_scale, w, dots_out, workspace,
-                              min_contig, cplx, stream, nullptr);
+                              min_contig, cplx, stream, nullptr, noise);
     }
     kind = kKernelLds;  // n within one tile, or prev dots wanted: the LDS kernel
     tile_log = 0;
     min_contig = 0;
   }
-  if (noise && kind != kKernelLds) {  // only the LDS kernel carries the noise term
+  if (noise && kind != kKernelLds) {  // the LDS kernel carries the noise term below one btile
     kind = kKernelLds;
     tile_log = 0;
     min_contig = 0;

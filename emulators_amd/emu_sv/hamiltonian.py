@@ -100,7 +100,8 @@ class RydbergHamiltonian:
 
     def _schedule(self, vec: torch.Tensor) -> autotune.Schedule:
         if self.noisy:
-            return _LDS
+            return autotune.best_noise_schedule(self.nqubits, vec.dtype, vec.device,
+                                                self.device_params(vec.device))
         return autotune.best_schedule(self.nqubits, vec.dtype, vec.device,
                                       self.device_params(vec.device), self.complex)
 

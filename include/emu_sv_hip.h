@@ -47,7 +47,9 @@ extern "C" {
 #define EMU_SV_FLAG_COMPLEX 1 /* some phi != 0: complex flip coefficients */
 #define EMU_SV_FLAG_NOISE 2   /* the block carries the non-Hermitian term   */
                               /* -i/2 sum L^+L (emu_sv_pack_noise); implies */
-                              /* complex coefficients and the LDS kernel    */
+                              /* complex coefficients; the LDS kernel, or   */
+                              /* the btile kernel for bare H.psi above one  */
+                              /* big tile (other kinds run the LDS kernel)  */
 /* The first pass's diagonal constants are tabulated in the workspace by a
  * small kernel before the passes.  With EMU_SV_FLAG_KEEP_DIAG (emu_sv_hpsi,
  * emu_sv_lanczos_matvec, emu_sv_lanczos_matvec_dev) the table is reused when

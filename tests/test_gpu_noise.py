@@ -317,3 +317,105 @@ def test_noisy_backend_run(gpu_device):
     occ = torch.as_tensor(res.get_result(cfg.observables[0].tag, 1.0))
     assert occ.shape == (10,)
     assert torch.all(occ >= -1e-12) and torch.all(occ <= 1 + 1e-12)
+
+
+@pytest.mark.parametrize("n,dtype,cmin", [(15, torch.complex128, 2), (16, torch.complex128, 4),
+                                          (18, torch.complex128, 2), (16, torch.complex64, 4),
+                                          (17, torch.complex64, 5), (19, torch.complex64, 4)])
+@pytest.mark.parametrize("phase", [False, True])
+def test_noisy_hpsi_btile_vs_oracle(gpu_device, n, dtype, cmin, phase):
+    """The big-tile kernel with the noise term (bare H.psi above one tile of
+    2^14 complex128 / 2^15 complex64 amplitudes) against the oracle's
+    index_add restatement of the noisy H (1e-12 relative in complex128,
+    2e-5 in complex64), and the fused dots of its last pass against the
+    output."""
+    from emulators_amd import _native as native
+    from emulators_amd.emu_sv.hamiltonian import RydbergHamiltonian
+    print(f"noisy btile n={n} {dtype} cmin={cmin} phase={phase}", flush=True)
+    om, de, ph, u = _params(n, 70 + n, phase)
+    k = orc.noise_from_lindbladians(LINDBLADS)
+    h = RydbergHamiltonian(om, de, ph, u, device=gpu_device, noise=k)
+    torch.manual_seed(n + 1)
+    psi = torch.randn(2**n, dtype=torch.complex128)
+    psi /= psi.norm()
+    sigma = 0.3 - 0.7j
+    expected = sigma * (orc.OracleHamiltonian(om, de, ph, u, noise=k) * psi)
+    v = psi.to(gpu_device, dtype)
+    w = torch.empty_like(v)
+    dots = native.hpsi(n, h.device_params(gpu_device), sigma, v, 1.0, w, dots=True, complex_coef=True,
+                       noise=True, kernel="btile", min_contig=cmin).cpu()
+    got = w.cpu().to(torch.complex128)
+    rel = float((got - expected).abs().max()) / float(expected.abs().max())
+    assert rel <= (1e-12 if dtype == torch.complex128 else 2e-5), rel
+    wc = got
+    vc = v.cpu().to(torch.complex128)
+    tol = 1e-12 if dtype == torch.complex128 else 1e-5
+    assert abs(float(dots[0]) - float(wc.norm() ** 2)) <= tol * float(wc.norm() ** 2)
+    ov = complex(torch.vdot(vc, wc))
+    assert abs(complex(float(dots[1]), float(dots[2])) - ov) <= tol * float(wc.norm())
+    # the LDS kernel with the noise term gives the same vector
+    w2 = torch.empty_like(v)
+    native.hpsi(n, h.device_params(gpu_device), sigma, v, 1.0, w2, complex_coef=True, noise=True,
+                kernel="lds")
+    d = float((w2.cpu().to(torch.complex128) - got).abs().max()) / float(expected.abs().max())
+    assert d <= (1e-13 if dtype == torch.complex128 else 2e-6), d
+
+
+_MC16 = {"n": 16, "steps": 2, "dt": 0.04, "tol": 1e-10, "seed": 11}
+
+
+def _mc16_case():
+    c = _MC16
+    om, de, ph, u = _params(c["n"], c["seed"], False)
+    lind = [math.sqrt(2.0) * torch.tensor([[0, 1], [0, 0]], dtype=torch.complex128),
+            math.sqrt(0.5) * torch.tensor([[1, 0], [0, -1]], dtype=torch.complex128)]
+    g = torch.Generator().manual_seed(c["seed"])
+    psi0 = torch.randn(2 ** c["n"], dtype=torch.complex128, generator=g)
+    return om, de, ph, u, lind, psi0 / psi0.norm()
+
+
+_mc16_ref: dict = {}
+
+
+def _mc16_oracle():
+    """The oracle's states after each step, its jumps and final threshold
+    (computed once per session: about 15 s of CPU)."""
+    if not _mc16_ref:
+        c = _MC16
+        om, de, ph, u, lind, psi0 = _mc16_case()
+        state = random.getstate()
+        random.seed(c["seed"])
+        ref = orc.OracleMonteCarlo()
+        states, psi = [], psi0.clone()
+        for _ in range(c["steps"]):
+            psi = ref.step(c["dt"], om, de, ph, u, psi, c["tol"], lind)
+            states.append(psi.clone())
+        random.setstate(state)
+        _mc16_ref.update(states=states, jumps=list(ref.jumps), threshold=ref.jump_threshold)
+    return _mc16_ref
+
+
+@pytest.mark.usefixtures("restore_rng_state")
+@pytest.mark.parametrize("schedule", ["c128=btile:0:2", "c128=lds:0:0", ""])
+def test_monte_carlo_step_n16_vs_oracle(gpu_device, monkeypatch, schedule):
+    """Monte-Carlo steps of a 16-atom register on the tuned noisy kernels
+    (big tiles, the LDS kernel, and the autotuner's own pick) against the
+    oracle's quantum-jump stepper: a strong decay from a random state, so
+    that the seeded threshold is crossed in the first step (Brent search,
+    jump draws, renormalisation); the same states within 1e-8 and the same
+    final threshold."""
+    from emulators_amd.emu_sv.time_evolution import EvolveMonteCarlo
+    monkeypatch.setenv("EMU_SV_SCHEDULE", schedule)
+    c = _MC16
+    print(f"monte carlo n=16 schedule={schedule!r}", flush=True)
+    ref = _mc16_oracle()
+    assert len(ref["jumps"]) >= 1  # the case exercises a jump
+    om, de, ph, u, lind, psi0 = _mc16_case()
+    random.seed(c["seed"])
+    mc = EvolveMonteCarlo()
+    psi = psi0.to(gpu_device)
+    for s in range(c["steps"]):
+        psi, _ = mc.apply(c["dt"], om, de, ph, u, psi, c["tol"], lind)
+        diff = float((psi.cpu() - ref["states"][s]).abs().max())
+        assert diff < 1e-8, f"step {s}: {diff}"
+    assert mc.jump_threshold == pytest.approx(ref["threshold"], rel=1e-9)
