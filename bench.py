@@ -516,6 +516,19 @@ def noise_legs() -> dict:
                     "(100 steps), relaxation 1/us + dephasing 0.5/us, complex128",
         "trajectories": len(times), "seconds_per_trajectory": sum(times[1:]) / (len(times) - 1),
         "first_trajectory_s": times[0]}
+    # a 16-atom trajectory: the tuned noisy H.psi kernels and the Gram-form
+    # Gram-Schmidt (seed 0; a short warm run first, which also tunes)
+    seq16 = workloads.noisy_chain(16, lind, stochastic=True)
+    random.seed(0)
+    SVBackend._run_from_sequence_data(workloads.noisy_chain(16, lind, True, duration=50), cfg)
+    random.seed(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = SVBackend._run_from_sequence_data(seq16, cfg)
+    torch.cuda.synchronize()
+    out["monte_carlo_n16"] = {
+        "workload": "16-atom chain, the same pulse and noise (100 steps), one trajectory (seed 0), "
+                    "complex128", "seconds_per_trajectory": time.perf_counter() - t0}
     n = 6
     seq = workloads.noisy_chain(n, lind, stochastic=False)
     SVBackend._run_from_sequence_data(workloads.noisy_chain(n, lind, False, duration=50), cfg)
@@ -529,7 +542,8 @@ def noise_legs() -> dict:
         "workload": f"{n}-atom chain density matrix (2^{2 * n} elements), the same pulse and noise, "
                     "100 steps, complex128", "seconds": t_gpu}
     log(f"noise: {out['monte_carlo']['seconds_per_trajectory']:.2f} s per Monte-Carlo trajectory "
-        f"(N=10), {t_gpu:.2f} s for the N=6 density matrix")
+        f"(N=10), {out['monte_carlo_n16']['seconds_per_trajectory']:.2f} s (N=16), "
+        f"{t_gpu:.2f} s for the N=6 density matrix")
     return out
 
 

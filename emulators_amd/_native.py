@@ -73,6 +73,9 @@ SIGNATURES = {
                                           _vp]),
     "emu_sv_arnoldi_orthonormalize": (_i, [_i, _i, _i, ctypes.POINTER(_vp), _dp, _vp, _vp, _vp,
                                            _vp]),
+    "emu_sv_arnoldi_gram": (_i, [_i, _i, _i, ctypes.POINTER(_vp), _vp, _d, _vp, _i, _vp, _vp]),
+    "emu_sv_arnoldi_gram_state_doubles": (ctypes.c_size_t, [_i]),
+    "emu_sv_arnoldi_exp_records_doubles": (ctypes.c_size_t, [_i]),
     "emu_sv_arnoldi_exp": (_i, [_i, _i, _vp, _d, _d, _vp, _vp, _vp, _i, _d, _d, _vp, _vp, _vp, _i, _i,
                                 _i, _vp, ctypes.POINTER(_i)]),
     "emu_sv_apply_1q": (_i, [_i, _i, _vp, _i, _dp, _vp]),
@@ -457,6 +460,43 @@ def arnoldi_orthogonalize(n: int, basis: list[torch.Tensor], scales: list[float]
     return rec
 
 
+def arnoldi_gram_state(ld: int, device) -> torch.Tensor:
+    """A zeroed device state buffer for emu_sv_arnoldi_gram with leading
+    dimension ``ld`` (more than the largest basis count it will serve)."""
+    size = int(LIB.emu_sv_arnoldi_gram_state_doubles(ld))
+    if size <= 0:
+        raise ValueError("arnoldi_gram: ld must be at least 2")
+    return torch.zeros(size, dtype=torch.float64, device=device)
+
+
+def arnoldi_gram(n: int, basis: list[torch.Tensor], w: torch.Tensor, v0_scale: float,
+                 state: torch.Tensor, ld: int, records: torch.Tensor | None = None) -> torch.Tensor:
+    """Gram-Schmidt of w = op(u_j) (j = len(basis) - 1, u_j unscaled) against
+    the unnormalised basis a_k = s_k u_k (s_0 = ``v0_scale``, later scales
+    on the device in ``state``) by the Gram-matrix form of modified
+    Gram-Schmidt; w becomes u_{j+1} = s_j w - sum_k h_k a_k, unnormalised.
+    Returns the device (count+1, 4) records: row k < count = [., Re h_k,
+    Im h_k, 0] (T[k, j]), row 0 [0] = ||s_j w||^2, row count [0] = ||u_{j+1}||^2
+    (s_{j+1} = 1 / its root is stored in ``state``)."""
+    _require_device_vec(w, "arnoldi_gram")
+    count = len(basis)
+    size, dt = 1 << n, w.dtype
+    if count < 1 or w.numel() != size:
+        raise ValueError("arnoldi_gram: w must have 2**n elements and the basis one vector at least")
+    for v in basis:
+        if not (v.is_cuda and v.dtype == dt and v.numel() == size and v.is_contiguous()):
+            raise ValueError("arnoldi_gram: basis vectors must be contiguous device tensors matching w")
+    if state.dtype != torch.float64 or state.device != w.device or \
+            state.numel() < int(LIB.emu_sv_arnoldi_gram_state_doubles(ld)):
+        raise ValueError("arnoldi_gram: state must be a float64 device buffer of the state size for ld")
+    if records is None:
+        records = torch.empty(count + 1, 4, dtype=torch.float64, device=w.device)
+    arr = (ctypes.c_void_p * count)(*[v.data_ptr() for v in basis])
+    check(LIB.emu_sv_arnoldi_gram(n, dtype_code(dt), count, arr, _ptr(w), float(v0_scale), _ptr(state),
+                                  ld, _ptr(records), _stream(w.device)), "arnoldi_gram")
+    return records
+
+
 class _ArnoldiExpBuffers:
     """Per-(device, stream, n, dtype, max_dim) buffers of emu_sv_arnoldi_exp:
     the basis, the device records and their pinned host mirror."""
@@ -464,7 +504,9 @@ class _ArnoldiExpBuffers:
     _cache: dict = {}
 
     def __init__(self, device, n: int, dtype, max_dim: int):
-        rows = (max_dim * (max_dim + 2) + 1) * 4
+        rows = int(LIB.emu_sv_arnoldi_exp_records_doubles(max_dim))
+        if rows <= 0:
+            raise ValueError("arnoldi_exp: max_dim must be in [1, 256]")
         self.basis = torch.empty(max_dim << n, dtype=dtype, device=device)
         self.records = torch.zeros(rows, dtype=torch.float64, device=device)
         self.host = torch.zeros(rows, dtype=torch.float64, pin_memory=True)

@@ -150,10 +150,10 @@ def best_schedule(n: int, dtype: torch.dtype, device: torch.device, params: torc
 # LDS and big-tile kernels carry the noise term.  The Arnoldi iteration calls
 # bare H.psi (no orthogonalisation), so the candidates are timed on that.
 NOISE_CANDIDATES = {
-    torch.complex64: [Schedule("lds", 0, 0), Schedule("lds", 11, 3), Schedule("btile", 0, 4),
-                      Schedule("btile", 0, 5)],
-    torch.complex128: [Schedule("lds", 0, 0), Schedule("lds", 11, 2), Schedule("btile", 0, 2),
-                       Schedule("btile", 0, 4)],
+    torch.complex64: [Schedule("lds", 0, 0), Schedule("lds", 9, 2), Schedule("lds", 10, 3),
+                      Schedule("lds", 11, 3), Schedule("btile", 0, 4), Schedule("btile", 0, 5)],
+    torch.complex128: [Schedule("lds", 0, 0), Schedule("lds", 9, 2), Schedule("lds", 10, 2),
+                       Schedule("lds", 11, 2), Schedule("btile", 0, 2), Schedule("btile", 0, 4)],
 }
 NOISE_DEFAULT = Schedule("lds", 0, 0)
 _noise_cache: dict = {}

@@ -1121,6 +1121,357 @@ int launch_mgs_small(int n, int count, const void *const *vecs, const double *sc
   return hip_check(hipGetLastError(), "arnoldi_mgs_small launch");
 }
 
+// ---------------------------------------------------------------------------
+// Arnoldi Gram-Schmidt with the Gram matrix of the basis (emu_sv_arnoldi_gram).
+// The reference's modified Gram-Schmidt (krylov_exp.py:54-58 with
+// is_hermitian=False) subtracts the basis vectors a_k one at a time,
+// h_k = <a_k|w_k>, w_{k+1} = w_k - h_k a_k.  Expanding w_k gives
+//   h_k = c_k - sum_{i<k} h_i G(k, i),  c_k = <a_k|w>,  G(k, i) = <a_k|a_i>,
+// so the same coefficients follow from one sweep of overlaps and the Gram
+// matrix of the basis: the new row G(j, .) of a_j is reduced in the same
+// sweep (it reads a_j anyway), earlier rows come from earlier calls.  Per
+// call: K1 the overlaps (count + 1 vectors read), K2 the reduction and the
+// forward substitution (one workgroup), K3 w' = a-scaled w - sum h_k a_k
+// with ||w'||^2 (count + 1 read, 1 written), K4 the norm and the next
+// basis vector's scale 1 / ||w'||.  The basis is stored unnormalised
+// (u_k, a_k = s_k u_k) with the scales on the device, so no normalisation
+// pass is needed and the caller can enqueue the next matvec (on u_{j+1},
+// scale 1) before reading this call's records.
+// State (doubles, ld >= count + 1): [0, ld) scales s_k; G(k, i) (i < k) as
+// (re, im) at ld + 2 (k (k - 1) / 2 + i) (packed lower triangle); 2 ld K3
+// coefficients; 8 misc; then the partial sums, kGramBlocks rows of 4 ld + 4.
+// ---------------------------------------------------------------------------
+constexpr int kGramMaxCount = 256;
+constexpr int kGramBlocks = 256;
+constexpr int kGramEpt = 2;  // packets per thread and chunk of the overlap sweep
+struct GramVecs {
+  const void *u[kGramMaxCount];
+};
+__host__ __device__ constexpr size_t gram_off_g(int ld) { return (size_t)ld; }
+__host__ __device__ constexpr size_t gram_off_coef(int ld) { return (size_t)ld + (size_t)ld * (ld - 1); }
+__host__ __device__ constexpr size_t gram_tri(int k, int i) { return 2 * ((size_t)k * (k - 1) / 2 + i); }
+__host__ __device__ constexpr size_t gram_off_misc(int ld) { return gram_off_coef(ld) + 2 * (size_t)ld; }
+__host__ __device__ constexpr size_t gram_off_part(int ld) { return gram_off_misc(ld) + 8; }
+__host__ __device__ constexpr size_t gram_part_stride(int ld) { return 4 * (size_t)ld + 4; }
+__host__ __device__ constexpr size_t gram_state_doubles(int ld) {
+  return gram_off_part(ld) + (size_t)kGramBlocks * gram_part_stride(ld);
+}
+
+// K1: workgroup (x, y) sums, over its chunks of the vectors, <u_i|w> and
+// <u_i|u_j> (j = count - 1; w and u_j stay in registers per chunk) for the
+// kGramGroup basis vectors i of group y, and (y = 0) ||w||^2.  Partial row x
+// layout: 4 i + (0, 1) = Re, Im <u_i|w>; 4 i + (2, 3) = Re, Im <u_i|u_j>;
+// 4 count = ||w||^2.  The second grid dimension fills the GPU on the small
+// vectors of the noisy paths (2^14 .. 2^18 elements) without more chunks.
+constexpr int kGramGroup = 8;
+template <typename T>
+__global__ __launch_bounds__(kThreads) void arnoldi_gram_dots_kernel(uint64_t npk, int count,
+                                                                     const GramVecs vecs,
+                                                                     const void *wv, double *part,
+                                                                     uint64_t stride) {
+  using P = typename Packet<T>::type;
+  constexpr int E = kGramEpt, A = Packet<T>::amps, GI = kGramGroup;
+  __shared__ double s_acc[kWaves][GI][4];
+  __shared__ double s_w[kWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i0 = blockIdx.y * GI;
+  const int ni = min(GI, count - i0);
+  for (int v = threadIdx.x; v < kWaves * GI * 4; v += kThreads) (&s_acc[0][0][0])[v] = 0.0;
+  __syncthreads();
+  const P *w = reinterpret_cast<const P *>(wv);
+  const P *uj = reinterpret_cast<const P *>(vecs.u[count - 1]);
+  const uint64_t chunk = (uint64_t)kThreads * E;
+  const uint64_t nchunks = (npk + chunk - 1) / chunk;
+  P zero;
+  for (int k = 0; k < A; ++k) packet_set(zero, k, 0.0, 0.0);
+  double wn = 0.0;
+  for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    P wp[E], jp[E];
+    uint64_t xs[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      xs[e] = c * chunk + (uint64_t)e * kThreads + threadIdx.x;
+      wp[e] = xs[e] < npk ? w[xs[e]] : zero;
+      jp[e] = xs[e] < npk ? uj[xs[e]] : zero;
+    }
+    if (blockIdx.y == 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+#pragma unroll
+        for (int k = 0; k < A; ++k) {
+          const double r = packet_re(wp[e], k), m = packet_im(wp[e], k);
+          wn += r * r + m * m;
+        }
+    }
+    // the group's loads first, then its sums
+    P q[GI][E];
+#pragma unroll
+    for (int g = 0; g < GI; ++g) {
+      const int i = i0 + g;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        q[g][e] = jp[e];  // i = j (u_j is in registers) and the group's unused slots
+        if (g < ni && i != count - 1 && xs[e] < npk) q[g][e] = reinterpret_cast<const P *>(vecs.u[i])[xs[e]];
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < GI; ++g) {
+      if (g >= ni) break;
+      double a4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+#pragma unroll
+        for (int k = 0; k < A; ++k) {
+          const double qr = packet_re(q[g][e], k), qi = packet_im(q[g][e], k);
+          const double wr = packet_re(wp[e], k), wi = packet_im(wp[e], k);
+          const double jr = packet_re(jp[e], k), ji = packet_im(jp[e], k);
+          a4[0] += qr * wr + qi * wi;
+          a4[1] += qr * wi - qi * wr;
+          a4[2] += qr * jr + qi * ji;
+          a4[3] += qr * ji - qi * jr;
+        }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) a4[v] = wave_sum(a4[v]);
+      if (lane == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) s_acc[wave][g][v] += a4[v];
+      }
+    }
+  }
+  wn = wave_sum(wn);
+  if (lane == 0) s_w[wave] = wn;
+  __syncthreads();
+  double *row = part + (uint64_t)blockIdx.x * stride;
+  for (int v = threadIdx.x; v < 4 * ni; v += kThreads) {
+    double sm = 0.0;
+#pragma unroll
+    for (int q = 0; q < kWaves; ++q) sm += s_acc[q][v >> 2][v & 3];
+    row[4 * i0 + v] = sm;
+  }
+  if (blockIdx.y == 0 && threadIdx.x == 0) {
+    double sm = 0.0;
+#pragma unroll
+    for (int q = 0; q < kWaves; ++q) sm += s_w[q];
+    row[4 * count] = sm;
+  }
+}
+
+// K2 (one workgroup of 1024 threads): the partial rows summed in a fixed
+// order (four row slices of every value, each with its loads in flight
+// together, then the slices in order), c_k and the new Gram row, then the
+// forward substitution L h = c, L unit lower triangular with L(k, i) =
+// G(k, i).  The basis is orthonormal to rounding, so |L - I| is at rounding
+// level and Jacobi sweeps h <- c - (L - I) h, one thread per row, reach the
+// substitution's solution within rounding in two sweeps (the error shrinks
+// by |L - I| per sweep); three are run, on the Gram triangle staged in LDS
+// (up to 64 basis vectors).  A basis whose Gram matrix is off by more than
+// 1e-3 anywhere is solved by sequential substitution instead (thread 0).
+// Writes the records (row k: [0, Re h_k, Im h_k, 0], row 0 [0] = ||a-scaled
+// w||^2), the K3 coefficients h_k s_k, s_j (misc[0]) and G(j, .).
+constexpr int kGramSweeps = 3;
+constexpr int kGramSolveThreads = 1024;
+constexpr int kGramLdsCount = 64;  // Gram triangles up to this count staged in LDS
+__global__ __launch_bounds__(kGramSolveThreads) void arnoldi_gram_solve_kernel(int count, int nb, int ld,
+                                                                               double s0, double *state,
+                                                                               double *records) {
+  constexpr int NT = kGramSolveThreads, S = 4, V = NT / S;
+  __shared__ double sums[4 * kGramMaxCount + 1];
+  __shared__ double s_part[S][V];
+  __shared__ double s_c[kGramMaxCount][2];
+  __shared__ double s_h[2][kGramMaxCount][2];
+  __shared__ double s_s[kGramMaxCount];
+  __shared__ double s_G[kGramLdsCount * (kGramLdsCount - 1)];  // packed triangle, (re, im)
+  __shared__ int s_far;
+  double *sc = state;
+  double *G = state + gram_off_g(ld);
+  double *coef = state + gram_off_coef(ld);
+  double *misc = state + gram_off_misc(ld);
+  const double *part = state + gram_off_part(ld);
+  const uint64_t stride = gram_part_stride(ld);
+  const int tid = threadIdx.x;
+  const int nvals = 4 * count + 1;
+  const int j = count - 1;
+  const bool lds_g = count <= kGramLdsCount;
+  if (tid == 0) s_far = 0;
+  // earlier rows of the Gram triangle (written by earlier calls), staged
+  if (lds_g)
+    for (int e = tid; e < j * (j - 1); e += NT) s_G[e] = G[e];
+  for (int k = tid; k < count; k += NT) s_s[k] = k == 0 ? s0 : sc[k];
+  for (int v0 = 0; v0 < nvals; v0 += V) {
+    const int v = v0 + tid % V, sl = tid / V;
+    double acc = 0.0;
+    if (v < nvals) {
+#pragma unroll 8
+      for (int b = sl; b < nb; b += S) acc += part[(uint64_t)b * stride + v];
+    }
+    s_part[sl][tid % V] = acc;
+    __syncthreads();
+    if (sl == 0 && v < nvals) sums[v] = ((s_part[0][tid] + s_part[1][tid]) + s_part[2][tid]) + s_part[3][tid];
+    __syncthreads();
+  }
+  const double sj = s_s[j];
+  for (int k = tid; k < count; k += NT) {
+    const double f = s_s[k] * sj;
+    s_c[k][0] = f * sums[4 * k];
+    s_c[k][1] = f * sums[4 * k + 1];
+    s_h[0][k][0] = s_c[k][0];
+    s_h[0][k][1] = s_c[k][1];
+    if (k < j) {  // G(j, k) = <a_j|a_k> = conj(<a_k|a_j>)
+      const double gr = f * sums[4 * k + 2], gi = -f * sums[4 * k + 3];
+      G[gram_tri(j, k)] = gr;
+      G[gram_tri(j, k) + 1] = gi;
+      if (lds_g) {
+        s_G[gram_tri(j, k)] = gr;
+        s_G[gram_tri(j, k) + 1] = gi;
+      }
+    }
+  }
+  if (tid == 0) {
+    sc[0] = s0;
+    misc[0] = sj;
+  }
+  __syncthreads();
+  // G(k, i) from LDS when staged; row j from the sums otherwise
+  auto gram_at = [&](int k, int i, double &gr, double &gi) {
+    if (lds_g) {
+      gr = s_G[gram_tri(k, i)];
+      gi = s_G[gram_tri(k, i) + 1];
+    } else if (k == j) {
+      const double f = s_s[i] * sj;
+      gr = f * sums[4 * i + 2];
+      gi = -f * sums[4 * i + 3];
+    } else {
+      gr = G[gram_tri(k, i)];
+      gi = G[gram_tri(k, i) + 1];
+    }
+  };
+  int cur = 0;
+  for (int sw = 0; sw < kGramSweeps; ++sw) {
+    for (int k = tid; k < count; k += NT) {
+      double ar = 0.0, ai = 0.0;
+      for (int i = 0; i < k; ++i) {
+        double gr, gi;
+        gram_at(k, i, gr, gi);
+        if (sw == 0 && fabs(gr) + fabs(gi) > 1e-3) s_far = 1;
+        const double hr = s_h[cur][i][0], hi = s_h[cur][i][1];
+        ar += hr * gr - hi * gi;
+        ai += hr * gi + hi * gr;
+      }
+      s_h[cur ^ 1][k][0] = s_c[k][0] - ar;
+      s_h[cur ^ 1][k][1] = s_c[k][1] - ai;
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  if (s_far) {  // far from orthonormal: sequential substitution
+    if (tid == 0) {
+      for (int k = 0; k < count; ++k) {
+        double ar = 0.0, ai = 0.0;
+        for (int i = 0; i < k; ++i) {
+          double gr, gi;
+          gram_at(k, i, gr, gi);
+          ar += s_h[cur][i][0] * gr - s_h[cur][i][1] * gi;
+          ai += s_h[cur][i][0] * gi + s_h[cur][i][1] * gr;
+        }
+        s_h[cur][k][0] = s_c[k][0] - ar;
+        s_h[cur][k][1] = s_c[k][1] - ai;
+      }
+    }
+    __syncthreads();
+  }
+  for (int k = tid; k < count; k += NT) {
+    const double hr = s_h[cur][k][0], hi = s_h[cur][k][1];
+    records[4 * k] = k == 0 ? sj * sj * sums[4 * count] : 0.0;
+    records[4 * k + 1] = hr;
+    records[4 * k + 2] = hi;
+    records[4 * k + 3] = 0.0;
+    coef[2 * k] = hr * s_s[k];
+    coef[2 * k + 1] = hi * s_s[k];
+  }
+}
+
+// K3: w' = s_j w - sum_k coef_k u_k (FP64, rounded once on store) and the
+// per-workgroup ||w'||^2 of the stored value.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void arnoldi_gram_update_kernel(uint64_t npk, int count,
+                                                                       const GramVecs vecs,
+                                                                       void *wv, const double *state,
+                                                                       int ld, double *part) {
+  using P = typename Packet<T>::type;
+  constexpr int A = Packet<T>::amps;
+  __shared__ double s_cf[kGramMaxCount][2];
+  __shared__ double red[kWaves];
+  const double *coef = state + gram_off_coef(ld);
+  for (int k = threadIdx.x; k < count; k += kThreads) {
+    s_cf[k][0] = coef[2 * k];
+    s_cf[k][1] = coef[2 * k + 1];
+  }
+  const double sj = state[gram_off_misc(ld)];
+  __syncthreads();
+  P *w = reinterpret_cast<P *>(wv);
+  double nrm = 0.0;
+  const uint64_t step = (uint64_t)gridDim.x * kThreads;
+  for (uint64_t x = (uint64_t)blockIdx.x * kThreads + threadIdx.x; x < npk; x += step) {
+    const P z = w[x];
+    double zr[A], zi[A];
+#pragma unroll
+    for (int k = 0; k < A; ++k) {
+      zr[k] = sj * packet_re(z, k);
+      zi[k] = sj * packet_im(z, k);
+    }
+#pragma unroll 4
+    for (int i = 0; i < count; ++i) {
+      const P q = reinterpret_cast<const P *>(vecs.u[i])[x];
+      const double cr = s_cf[i][0], ci = s_cf[i][1];
+#pragma unroll
+      for (int k = 0; k < A; ++k) {
+        const double qr = packet_re(q, k), qi = packet_im(q, k);
+        zr[k] -= cr * qr - ci * qi;
+        zi[k] -= cr * qi + ci * qr;
+      }
+    }
+    P o;
+#pragma unroll
+    for (int k = 0; k < A; ++k) packet_set(o, k, zr[k], zi[k]);
+    w[x] = o;
+#pragma unroll
+    for (int k = 0; k < A; ++k) {
+      const double r = packet_re(o, k), m = packet_im(o, k);
+      nrm += r * r + m * m;
+    }
+  }
+  nrm = wave_sum(nrm);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = nrm;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sm = 0.0;
+#pragma unroll
+    for (int q = 0; q < kWaves; ++q) sm += red[q];
+    part[blockIdx.x] = sm;
+  }
+}
+
+// K4 (one workgroup): ||w'||^2 summed in a fixed order into records[0] and
+// the next basis vector's scale s_count = 1 / ||w'|| (0 for a zero w').
+__global__ __launch_bounds__(kThreads) void arnoldi_gram_norm_kernel(int nb, const double *part,
+                                                                     double *rec, double *scale_out) {
+  __shared__ double red[kThreads];
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kThreads) acc += part[b];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = kThreads / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double n2 = red[0];
+    rec[0] = n2;
+    rec[1] = rec[2] = rec[3] = 0.0;
+    *scale_out = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
+  }
+}
+
 // A 2x2 operator on one bit: thread p owns the pair (x0, x0 | 2^bit).
 struct Op1q {
   double a[8];  // A00, A01, A10, A11 as (re, im)
@@ -3076,6 +3427,55 @@ int arnoldi_mgs(int n, int dtype, int count, const void *const *vecs, const doub
   return hip_check(hipGetLastError(), "arnoldi normalisation launch");
 }
 
+size_t emu_sv_arnoldi_gram_state_doubles(int ld) {
+  return ld < 2 ? 0 : gram_state_doubles(ld);
+}
+
+int emu_sv_arnoldi_gram(int n, int dtype, int count, const void *const *vecs, void *w,
+                        double v0_scale, double *state, int ld, double *records_out,
+                        hipStream_t stream) {
+  if (int rc = check_n(n)) return rc;
+  if (int rc = check_dtype(dtype)) return rc;
+  if (count < 1 || count > kGramMaxCount)
+    return fail(EMU_SV_EINVAL, "arnoldi_gram: count must be in [1, 256]");
+  if (ld <= count) return fail(EMU_SV_EINVAL, "arnoldi_gram: ld must exceed count");
+  if (!vecs || !w || !state || !records_out) return fail(EMU_SV_EINVAL, "arnoldi_gram: null pointer");
+  GramVecs gv;
+  for (int k = 0; k < count; ++k) {
+    if (!vecs[k]) return fail(EMU_SV_EINVAL, "arnoldi_gram: null basis vector");
+    if (vecs[k] == w) return fail(EMU_SV_EINVAL, "arnoldi_gram: w aliases a basis vector");
+    gv.u[k] = vecs[k];
+  }
+  const uint64_t npk = packets(n, dtype);
+  const uint64_t chunk = (uint64_t)kThreads * kGramEpt;
+  const uint64_t nchunks = (npk + chunk - 1) / chunk;
+  const int nb1 = (int)(nchunks < (uint64_t)kGramBlocks ? nchunks : (uint64_t)kGramBlocks);
+  const dim3 grid1((unsigned)nb1, (unsigned)((count + kGramGroup - 1) / kGramGroup));
+  double *part = state + gram_off_part(ld);
+  const uint64_t stride = gram_part_stride(ld);
+  if (dtype == EMU_SV_C128)
+    hipLaunchKernelGGL(arnoldi_gram_dots_kernel<double>, grid1, dim3(kThreads), 0, stream, npk,
+                       count, gv, w, part, stride);
+  else
+    hipLaunchKernelGGL(arnoldi_gram_dots_kernel<float>, grid1, dim3(kThreads), 0, stream, npk,
+                       count, gv, w, part, stride);
+  if (int rc = hip_check(hipGetLastError(), "arnoldi_gram overlaps launch")) return rc;
+  hipLaunchKernelGGL(arnoldi_gram_solve_kernel, dim3(1), dim3(kGramSolveThreads), 0, stream, count,
+                     nb1, ld, v0_scale, state, records_out);
+  if (int rc = hip_check(hipGetLastError(), "arnoldi_gram solve launch")) return rc;
+  const int nb3 = stream_blocks(npk);
+  if (dtype == EMU_SV_C128)
+    hipLaunchKernelGGL(arnoldi_gram_update_kernel<double>, dim3(nb3), dim3(kThreads), 0, stream,
+                       npk, count, gv, w, state, ld, part);
+  else
+    hipLaunchKernelGGL(arnoldi_gram_update_kernel<float>, dim3(nb3), dim3(kThreads), 0, stream,
+                       npk, count, gv, w, state, ld, part);
+  if (int rc = hip_check(hipGetLastError(), "arnoldi_gram update launch")) return rc;
+  hipLaunchKernelGGL(arnoldi_gram_norm_kernel, dim3(1), dim3(kThreads), 0, stream, nb3, part,
+                     records_out + 4 * count, state + count);
+  return hip_check(hipGetLastError(), "arnoldi_gram norm launch");
+}
+
 int emu_sv_arnoldi_orthogonalize(int n, int dtype, int count, const void *const *vecs,
                                  const double *scales, void *w, double *records, void *workspace,
                                  hipStream_t stream) {
@@ -3106,6 +3506,11 @@ struct EventPair {
 };
 }  // namespace
 
+size_t emu_sv_arnoldi_exp_records_doubles(int max_dim) {
+  if (max_dim < 1 || max_dim > kArnoldiMaxDim) return 0;
+  return ((size_t)max_dim * (max_dim + 2) + 1) * 4 + gram_state_doubles(max_dim + 1);
+}
+
 int emu_sv_arnoldi_exp(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
                        void *v, void *out, void *basis, int max_dim, double exp_tolerance,
                        double norm_tolerance, double *records, double *host_records,
@@ -3122,6 +3527,11 @@ int emu_sv_arnoldi_exp(int n, int dtype, const double *ham, double sigma_re, dou
   const size_t row = (size_t)(max_dim + 2) * 4;  // doubles per iteration's records
   double *dot_dev = records + (size_t)max_dim * row;
   double *dot_host = host_records + (size_t)max_dim * row;
+  // Gram-Schmidt: the one-workgroup kernel for 2^10..2^13 elements, else the
+  // Gram-matrix form (unnormalised basis, scales on the device)
+  const bool use_gram = !(n >= 10 && n <= kMgsSmallLog && max_dim <= kMgsSmallCount);
+  const int gld = max_dim + 1;
+  double *gstate = dot_dev + 4;
   const int hflags = flags & ~EMU_SV_FLAG_KEEP_DIAG;
   EventPair ev;
   for (hipEvent_t &x : ev.e)
@@ -3143,13 +3553,18 @@ int emu_sv_arnoldi_exp(int n, int dtype, const double *ham, double sigma_re, dou
   auto enqueue = [&](int j) -> int {
     void *w = static_cast<char *>(basis) + (size_t)j * vbytes;
     t_diag_known_fresh = j > 0;
-    const int hrc = hpsi_impl(n, dtype, ham, sigma_re, sigma_im, vec[j], sc[j], nullptr, 0.0, w, nullptr,
-                              workspace, tile_log, min_contig, hflags | (j > 0 ? EMU_SV_FLAG_KEEP_DIAG : 0),
-                              stream);
+    const int hrc = hpsi_impl(n, dtype, ham, sigma_re, sigma_im, vec[j], use_gram ? 1.0 : sc[j], nullptr,
+                              0.0, w, nullptr, workspace, tile_log, min_contig,
+                              hflags | (j > 0 ? EMU_SV_FLAG_KEEP_DIAG : 0), stream);
     t_diag_known_fresh = false;
     if (hrc) return hrc;
     double *rec = records + (size_t)j * row;
-    if (int rc = arnoldi_mgs(n, dtype, j + 1, vec.data(), sc.data(), w, rec, workspace, 1, stream)) return rc;
+    if (use_gram) {
+      if (int rc = emu_sv_arnoldi_gram(n, dtype, j + 1, vec.data(), w, sc[0], gstate, gld, rec, stream))
+        return rc;
+    } else if (int rc = arnoldi_mgs(n, dtype, j + 1, vec.data(), sc.data(), w, rec, workspace, 1, stream)) {
+      return rc;
+    }
     if (int rc = hip_check(hipMemcpyAsync(host_records + (size_t)j * row, rec, (size_t)(j + 2) * 4 * sizeof(double),
                                           hipMemcpyDeviceToHost, stream), "arnoldi_exp: copy"))
       return rc;
@@ -3203,9 +3618,13 @@ int emu_sv_arnoldi_exp(int n, int dtype, const double *ham, double sigma_re, dou
     if (int rc = hip_check(hipEventSynchronize(ev.e[j & 1]), "arnoldi_exp: wait")) return rc;
     const double *rec = host_records + (size_t)j * row;
     const double nrm = std::sqrt(std::max(rec[0], 0.0));  // ||op(v_j)||, before orthogonalisation
-    for (int k = 0; k <= j; ++k) T[(size_t)k * dim + j] = sc[k] * cd(rec[4 * k + 1], rec[4 * k + 2]);
+    // Gram form: the records hold T[k, j] itself, and the next (unnormalised)
+    // basis vector's scale is 1 / n2
+    for (int k = 0; k <= j; ++k)
+      T[(size_t)k * dim + j] = (use_gram ? 1.0 : sc[k]) * cd(rec[4 * k + 1], rec[4 * k + 2]);
     const double n2 = std::sqrt(std::max(rec[4 * (j + 1)], 0.0));
     T[(size_t)(j + 1) * dim + j] = n2;
+    if (use_gram) sc[j + 1] = n2 > 0.0 ? 1.0 / n2 : 0.0;
     if (n2 < norm_tolerance) {  // happy breakdown
       if (int rc = expm_col(j + 1)) return rc;
       return finish(j + 1, j + 1, 1, 1);

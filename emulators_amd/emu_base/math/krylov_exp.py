@@ -433,18 +433,27 @@ class _ArnoldiBuffers:
 
     _cache: dict = {}
 
-    def __init__(self, rows: int):
+    def __init__(self, rows: int, device: torch.device):
         self.host = torch.zeros((rows, rows + 2, 4), dtype=torch.float64, pin_memory=True)
         self.events = [torch.cuda.Event() for _ in range(rows)]
         self.rows = rows
+        self.ld = rows + 1  # Gram-form state: up to rows basis vectors plus the next
+        self.gram_state = native.arnoldi_gram_state(self.ld, device)
 
     @classmethod
     def get(cls, device: torch.device, rows: int) -> "_ArnoldiBuffers":
         key = native.stream_key(device)
         b = cls._cache.get(key)
         if b is None or b.rows < rows:
-            b = cls._cache[key] = cls(rows)
+            b = cls._cache[key] = cls(rows, device)
         return b
+
+
+def _gram_schmidt_in_one_workgroup(n: int, max_krylov_dim: int) -> bool:
+    """The one-workgroup modified Gram-Schmidt kernel serves 2^10..2^13
+    elements (up to 128 basis vectors); elsewhere the pipelined engines use
+    the Gram-matrix form (emu_sv_arnoldi_gram)."""
+    return 10 <= n <= 13 and max_krylov_dim <= 128
 
 
 def _pipelined_arnoldi(op, v: torch.Tensor, exp_tolerance: float, norm_tolerance: float,
@@ -458,22 +467,32 @@ def _pipelined_arnoldi(op, v: torch.Tensor, exp_tolerance: float, norm_tolerance
     ahead when iteration j is likely to converge (half the last error
     estimate below the tolerance, as in the Lanczos pipeline); a discarded
     speculative iteration costs one matvec.  Same arithmetic as
-    ``_fused_arnoldi`` apart from the rounding of the stored unit vectors."""
+    ``_fused_arnoldi`` apart from the rounding of the stored unit vectors.
+
+    Outside the one-workgroup range the Gram-Schmidt step is the Gram-matrix
+    form (emu_sv_arnoldi_gram): the same coefficients from one overlap sweep
+    and one subtraction sweep, with the new basis vector left unnormalised
+    and its scale kept on the device (the records then hold T[k, j]
+    itself)."""
     shape = v.shape
     v = v.contiguous().view(-1)
     n = native.nqubits_of(v)
     matvec = _arnoldi_matvec(op, v, n)
     initial_norm = math.sqrt(float(native.dot(n, v)[0].item()))
     basis = [v]
-    scales = [1.0 / initial_norm]  # later basis vectors are unit vectors
+    scales = [1.0 / initial_norm]  # later basis vectors: unit (MGS) or 1 / ||u|| (Gram form)
     T = np.zeros((max_krylov_dim + 2, max_krylov_dim + 2), dtype=np.complex128)
     bufs = _ArnoldiBuffers.get(v.device, max_krylov_dim)
+    gram = not _gram_schmidt_in_one_workgroup(n, max_krylov_dim)
 
     def enqueue(j: int) -> None:
         w = torch.empty_like(v)
-        matvec(basis[j], scales[j], w)
+        matvec(basis[j], 1.0 if gram else scales[j], w)
         KrylovStats.matvecs += 1
-        rec = native.arnoldi_orthogonalize(n, basis[: j + 1], scales[: j + 1], w, normalize=True)
+        if gram:
+            rec = native.arnoldi_gram(n, basis[: j + 1], w, scales[0], bufs.gram_state, bufs.ld)
+        else:
+            rec = native.arnoldi_orthogonalize(n, basis[: j + 1], scales[: j + 1], w, normalize=True)
         bufs.host[j, : j + 2].copy_(rec, non_blocking=True)
         bufs.events[j].record()
         basis.append(w)
@@ -494,9 +513,11 @@ def _pipelined_arnoldi(op, v: torch.Tensor, exp_tolerance: float, norm_tolerance
         rec = bufs.host[j, : j + 2].numpy()
         nrm = math.sqrt(max(rec[0, 0], 0.0))  # ||op(v_j)||, before orthogonalisation
         for k in range(j + 1):
-            T[k, j] = scales[k] * complex(rec[k, 1], rec[k, 2])
+            T[k, j] = (1.0 if gram else scales[k]) * complex(rec[k, 1], rec[k, 2])
         n2 = math.sqrt(max(rec[j + 1, 0], 0.0))
         T[j + 1, j] = n2
+        if gram and n2 > 0.0:
+            scales[j + 1] = 1.0 / n2
         if n2 < norm_tolerance:  # happy breakdown
             expd = native.expm_column(T[: j + 1, : j + 1])[:, None]
             return KrylovExpResult(combine(expd[:, 0]), True, True, j + 1)

@@ -285,6 +285,35 @@ int emu_sv_arnoldi_orthonormalize(int n, int dtype, int count, const void *const
                                   const double *scales, void *w, double *records_out,
                                   void *workspace, hipStream_t stream);
 
+/* The Arnoldi Gram-Schmidt step in its Gram-matrix form, for the pipelined
+ * engines on vectors outside the one-workgroup range.  Replaces the same
+ * loop as emu_sv_arnoldi_orthogonalize (emu_base/math/krylov_exp.py:54-61
+ * with is_hermitian=False): the reference's modified Gram-Schmidt
+ * coefficients h_k = <a_k|w_k> equal c_k - sum_{i<k} h_i G(k, i) with
+ * c_k = <a_k|w> and G(k, i) = <a_k|a_i>, so one sweep of overlaps (which
+ * also reduces the new Gram row of a_j) and one subtraction sweep replace
+ * the 2 (count + 1) launches of the chain: 2 count + 3 vector transfers
+ * instead of 4 count + 6.  The basis is kept unnormalised, a_k = s_k u_k,
+ * with the scales on the device, so the next matvec can be enqueued on
+ * u_{j+1} with scale 1 before this call's records are read.
+ *   vecs     u_0 .. u_j (count = j + 1 <= 256 device vectors of 2^n elements)
+ *   w        in: op(u_j) (unscaled); out: u_{j+1} = s_j w - sum_k h_k a_k
+ *   v0_scale s_0 (1 / ||u_0|| for the Krylov start vector)
+ *   state    device doubles, emu_sv_arnoldi_gram_state_doubles(ld): the
+ *            scales, the Gram matrix of the basis and scratch; ld > count.
+ *            Needs no clearing: every entry a call reads was written by an
+ *            earlier call of the same run (or by this one)
+ *   records_out  device (count + 1) x 4 doubles: row k < count =
+ *            [0, Re h_k, Im h_k, 0] (the Hessenberg column T[k, j]), with
+ *            row 0 [0] = ||s_j w||^2; row count = [||u_{j+1}||^2, 0, 0, 0]
+ *            (T[j+1, j] is its root; s_{j+1} = 1 / root is kept in state,
+ *            0 when it is zero).
+ * Calls of one run go in order count = 1, 2, ... on one stream. */
+size_t emu_sv_arnoldi_gram_state_doubles(int ld);
+int emu_sv_arnoldi_gram(int n, int dtype, int count, const void *const *vecs, void *w,
+                        double v0_scale, double *state, int ld, double *records_out,
+                        hipStream_t stream);
+
 /* out = exp(sigma H) v by the Arnoldi branch of krylov_exp
  * (emu_base/math/krylov_exp.py:23-107 with is_hermitian=False) with its loop
  * inside the library: per iteration emu_sv_hpsi (flags: kernel kind,
@@ -298,11 +327,15 @@ int emu_sv_arnoldi_orthonormalize(int n, int dtype, int count, const void *const
  * iteration costs one matvec.  Synchronises the stream.
  *   v          device input state (2^n elements, not modified; out may alias it)
  *   basis      device buffer of max_dim vectors of 2^n elements (v_1 .. v_max_dim)
- *   records    device, (max_dim * (max_dim + 2) + 1) * 4 doubles
+ *   records    device, emu_sv_arnoldi_exp_records_doubles(max_dim) doubles
+ *              (the per-iteration records, then the Gram-Schmidt state)
  *   host_records  pinned host buffer of the same size
+ * The Gram-Schmidt step is the one-workgroup emu_sv_arnoldi_orthonormalize
+ * for 10 <= n <= 13 (max_dim <= 128), else emu_sv_arnoldi_gram.
  *   result_out host int[4]: iterations, converged, happy breakdown, matvecs
  * max_dim in [1, 256].  Returns EMU_SV_OK also when the iteration did not
  * converge (result_out[1] == 0: the reference's RecursionError). */
+size_t emu_sv_arnoldi_exp_records_doubles(int max_dim);
 int emu_sv_arnoldi_exp(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
                        void *v, void *out, void *basis, int max_dim, double exp_tolerance,
                        double norm_tolerance, double *records, double *host_records,

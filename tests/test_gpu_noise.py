@@ -132,6 +132,54 @@ def test_arnoldi_orthonormalize(gpu_device, n, count, dtype):
     assert float(z.abs().max()) == 0.0
 
 
+@pytest.mark.parametrize("n,steps", [(3, 6), (9, 12), (14, 10), (16, 24)])
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_arnoldi_gram_vs_mgs(gpu_device, n, steps, dtype):
+    """emu_sv_arnoldi_gram against the modified Gram-Schmidt launch chain
+    over a run of ``steps`` calls on the same inputs: each call's Hessenberg
+    column T[:, j] (records) and norms, and the new basis vector (unnormalised
+    u_{j+1} times its device scale against the chain's unit vector), within
+    1e-12 (complex128) / 2e-5 (complex64) relative; the scaled basis of the
+    Gram form stays orthonormal to 1e-13 (complex128)."""
+    from emulators_amd import _native as native
+    print(f"arnoldi gram n={n} steps={steps} {dtype}", flush=True)
+    g = torch.Generator().manual_seed(500 + n)
+    size = 2**n
+    v0 = torch.randn(size, dtype=torch.complex128, generator=g).to(gpu_device, dtype)
+    s0 = 1.0 / math.sqrt(float(native.dot(n, v0)[0]))
+    ld = steps + 2
+    state = native.arnoldi_gram_state(ld, gpu_device)
+    gbasis, gscale = [v0], [s0]
+    mbasis, mscale = [v0], [s0]
+    tol = 1e-12 if dtype == torch.complex128 else 2e-5
+    for j in range(min(steps, size - 1)):
+        r = torch.randn(size, dtype=torch.complex128, generator=g).to(gpu_device, dtype)
+        wg = r.clone()
+        recg = native.arnoldi_gram(n, gbasis, wg, s0, state, ld).cpu().numpy()
+        # the chain sees the same vector, scaled as the Gram form scales it
+        wm = (r.to(torch.complex128) * gscale[j]).to(dtype)
+        recm = native.arnoldi_orthogonalize(n, mbasis, mscale, wm, normalize=True).cpu().numpy()
+        colm = np.array([mscale[k] * complex(recm[k, 1], recm[k, 2]) for k in range(j + 1)])
+        colg = recg[: j + 1, 1] + 1j * recg[: j + 1, 2]
+        ref = max(1.0, float(np.abs(colm).max()))
+        assert float(np.abs(colg - colm).max()) <= tol * ref, (j, np.abs(colg - colm).max())
+        assert abs(recg[0, 0] - recm[0, 0]) <= tol * recm[0, 0]
+        assert abs(recg[j + 1, 0] - recm[j + 1, 0]) <= tol * max(recm[0, 0], 1.0)
+        n2 = math.sqrt(recg[j + 1, 0])
+        unit_g = wg.to(torch.complex128) / n2
+        diff = float((unit_g - wm.to(torch.complex128)).abs().max())
+        assert diff <= (1e-11 if dtype == torch.complex128 else 1e-4), (j, diff)
+        gbasis.append(wg)
+        gscale.append(1.0 / n2)
+        mbasis.append(wm)
+        mscale.append(1.0)
+    if dtype == torch.complex128:
+        a = torch.stack([b.to(torch.complex128) * s for b, s in zip(gbasis, gscale)])
+        gram = a.conj() @ a.T
+        eye = torch.eye(len(gbasis), dtype=torch.complex128, device=gpu_device)
+        assert float((gram - eye).abs().max()) < 1e-13
+
+
 @pytest.mark.parametrize("n,dt,tol", [(8, 0.5, 1e-10), (10, 1.0, 1e-8), (14, 0.3, 1e-10)])
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
 def test_arnoldi_engines_agree(gpu_device, monkeypatch, n, dt, tol, dtype):
