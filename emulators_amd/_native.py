@@ -73,6 +73,9 @@ SIGNATURES = {
                                           _vp]),
     "emu_sv_arnoldi_orthonormalize": (_i, [_i, _i, _i, ctypes.POINTER(_vp), _dp, _vp, _vp, _vp,
                                            _vp]),
+    "emu_sv_krylov_step_records_doubles": (ctypes.c_size_t, [_i]),
+    "emu_sv_krylov_step": (_i, [_i, _i, _vp, _d, _d, _vp, _vp, _vp, _i, _d, _d, _vp, _vp, _vp, _i, _i,
+                                _i, _vp, ctypes.POINTER(_i)]),
     "emu_sv_arnoldi_gram": (_i, [_i, _i, _i, ctypes.POINTER(_vp), _vp, _d, _vp, _i, _vp, _vp]),
     "emu_sv_arnoldi_gram_state_doubles": (ctypes.c_size_t, [_i]),
     "emu_sv_arnoldi_exp_records_doubles": (ctypes.c_size_t, [_i]),
@@ -541,6 +544,55 @@ def arnoldi_exp(n: int, ham_dev: torch.Tensor, sigma: complex, v: torch.Tensor, 
                                  float(norm_tolerance), _ptr(bufs.records), _ptr(bufs.host),
                                  _ptr(ws.scratch), tile_log, min_contig, flags, _stream(v.device), res),
           "arnoldi_exp")
+    return int(res[0]), bool(res[1]), bool(res[2]), int(res[3])
+
+
+class _KrylovStepBuffers:
+    """Per-(device, stream, max_dim) device records and their pinned host
+    mirror for emu_sv_krylov_step (the basis is allocated per call)."""
+
+    _cache: dict = {}
+
+    def __init__(self, device, max_dim: int):
+        size = int(LIB.emu_sv_krylov_step_records_doubles(max_dim))
+        if size <= 0:
+            raise ValueError("krylov_step: max_dim must be in [1, 256]")
+        self.records = torch.zeros(size, dtype=torch.float64, device=device)
+        self.host = torch.zeros(size, dtype=torch.float64, pin_memory=True)
+
+    @classmethod
+    def get(cls, device, max_dim: int) -> "_KrylovStepBuffers":
+        key = (stream_key(device), max_dim)
+        b = cls._cache.get(key)
+        if b is None:
+            b = cls._cache[key] = cls(device, max_dim)
+        return b
+
+
+def krylov_step(n: int, ham_dev: torch.Tensor, sigma: complex, v: torch.Tensor, out: torch.Tensor,
+                max_dim: int, exp_tolerance: float, norm_tolerance: float, complex_coef: bool = True,
+                tile_log: int = 0, min_contig: int = 0, kernel: str = "lds", partners: int = 0,
+                first_log: int = 0) -> tuple[int, bool, bool, int]:
+    """out = exp(sigma H) v by the library's pipelined Lanczos driver
+    (emu_sv_krylov_step); returns (iterations, converged, happy breakdown,
+    matvecs).  The basis (max_dim + 1 vectors) comes from torch's allocator
+    for the call."""
+    _require_device_vec(v, "krylov_step")
+    _require_device_vec(out, "krylov_step")
+    if v.numel() != 1 << n or out.numel() != 1 << n or out.dtype != v.dtype:
+        raise ValueError("krylov_step: v and out must be 2**n elements of one dtype")
+    _require_doubles(ham_dev, EMU_SV_HAM_DOUBLES, "krylov_step (parameter block)")
+    bufs = _KrylovStepBuffers.get(v.device, max_dim)
+    basis = torch.empty((max_dim + 1) << n, dtype=v.dtype, device=v.device)
+    ws = Workspace.get(v.device, n)
+    flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags(kernel, partners, first_log)
+    res = (ctypes.c_int * 4)()
+    check(LIB.emu_sv_krylov_step(n, dtype_code(v.dtype), _ptr(ham_dev), float(sigma.real),
+                                 float(sigma.imag), _ptr(v), _ptr(out), _ptr(basis), max_dim,
+                                 float(exp_tolerance), float(norm_tolerance), _ptr(bufs.records),
+                                 _ptr(bufs.host), _ptr(ws.scratch), tile_log, min_contig, flags,
+                                 _stream(v.device), res), "krylov_step")
+    del basis
     return int(res[0]), bool(res[1]), bool(res[2]), int(res[3])
 
 

@@ -3642,6 +3642,163 @@ int emu_sv_arnoldi_exp(int n, int dtype, const double *ham, double sigma_re, dou
   return finish(max_dim + 1, max_dim, 0, 0);
 }
 
+// ---------------------------------------------------------------------------
+// One Krylov time step of the Hermitian (Lanczos) branch with its loop in the
+// library: the pipelined engine of the Python package in native code.
+// Records: rows of kStepRow doubles; row j (iteration j) holds [0, 5) the
+// orthogonalisation sums, [5, 12) the last pass's dots, [16, 22) the device
+// coefficients the iteration reads, [24, 30) the side outputs (n2, b, a,
+// ||r||^2 / n2^2 of iteration j+1); row max_dim + 2 the opening dot and
+// H.psi reductions.
+// ---------------------------------------------------------------------------
+constexpr int kStepRow = 32;
+
+size_t emu_sv_krylov_step_records_doubles(int max_dim) {
+  if (max_dim < 1 || max_dim > kArnoldiMaxDim) return 0;
+  return (size_t)kStepRow * (max_dim + 3);
+}
+
+int emu_sv_krylov_step(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
+                       const void *v, void *out, void *basis, int max_dim, double exp_tolerance,
+                       double norm_tolerance, double *records, double *host_records,
+                       void *workspace, int tile_log, int min_contig, int flags,
+                       hipStream_t stream, int *result_out) {
+  if (int rc = check_n(n)) return rc;
+  if (int rc = check_dtype(dtype)) return rc;
+  if (!ham || !v || !out || !basis || !records || !host_records || !workspace || !result_out)
+    return fail(EMU_SV_EINVAL, "krylov_step: null pointer");
+  if (max_dim < 1 || max_dim > kArnoldiMaxDim)
+    return fail(EMU_SV_EINVAL, "krylov_step: max_dim must be in [1, 256]");
+  if (flags & EMU_SV_FLAG_NOISE)
+    return fail(EMU_SV_EINVAL, "krylov_step: the Lanczos recurrence needs a Hermitian H (no noise term)");
+  const int kind = kernel_kind(flags);
+  if (kind != kKernelTile && kind != kKernelLds && kind != kKernelBtile)
+    return fail(EMU_SV_EINVAL, "krylov_step: needs the tile, btile or LDS kernel kind");
+  const size_t vbytes = (dtype == EMU_SV_C128 ? sizeof(double2) : sizeof(float2)) << n;
+  // basis row k (k >= 1) holds w_{k-1} and then, orthogonalised in place,
+  // basis vector k; rows 1 .. max_dim + 1
+  auto row_ptr = [&](int k) -> void * { return static_cast<char *>(basis) + (size_t)(k - 1) * vbytes; };
+  {
+    const char *o = static_cast<const char *>(out), *b = static_cast<const char *>(basis);
+    if (o >= b && o < b + (size_t)(max_dim + 1) * vbytes)
+      return fail(EMU_SV_EINVAL, "krylov_step: out must not alias the basis");
+  }
+  double *open_dev = records + (size_t)(max_dim + 2) * kStepRow;
+  double *open_host = host_records + (size_t)(max_dim + 2) * kStepRow;
+  const int base_flags = flags & ~EMU_SV_FLAG_KEEP_DIAG;
+  const std::complex<double> sig(sigma_re, sigma_im);
+
+  // initial norm and the opening H.psi with its dots: the two synchronisations
+  // outside the iterations
+  if (int rc = emu_sv_dot(n, dtype, v, nullptr, open_dev, workspace, stream)) return rc;
+  if (int rc = hip_check(hipMemcpyAsync(open_host, open_dev, 3 * sizeof(double), hipMemcpyDeviceToHost, stream),
+                         "krylov_step: copy")) return rc;
+  if (int rc = hip_check(hipStreamSynchronize(stream), "krylov_step: synchronize")) return rc;
+  const double initial_norm = std::sqrt(std::max(open_host[0], 0.0));
+  if (!(initial_norm > 0.0)) return fail(EMU_SV_EINVAL, "krylov_step: zero input vector");
+  const double s0 = 1.0 / initial_norm;
+  if (int rc = hpsi_impl(n, dtype, ham, sigma_re, sigma_im, v, s0, nullptr, 0.0, row_ptr(1), open_dev + 8,
+                         workspace, tile_log, min_contig, base_flags, stream)) return rc;
+  if (int rc = hip_check(hipMemcpyAsync(open_host + 8, open_dev + 8, 7 * sizeof(double),
+                                        hipMemcpyDeviceToHost, stream), "krylov_step: copy")) return rc;
+  if (int rc = hip_check(hipStreamSynchronize(stream), "krylov_step: synchronize")) return rc;
+  const std::complex<double> b0(open_host[9], open_host[10]);
+  double ww = open_host[8];  // ||op(v_0)||^2
+  // iteration 0's coefficients: a = 0, b = b0 (pb = v with scale s0), w_scale = 1
+  double *c0 = host_records + 16;
+  c0[0] = 0.0;
+  c0[1] = 0.0;
+  c0[2] = b0.real() * s0;
+  c0[3] = b0.imag() * s0;
+  c0[4] = s0;
+  c0[5] = 1.0;
+  if (int rc = hip_check(hipMemcpyAsync(records + 16, c0, 6 * sizeof(double), hipMemcpyHostToDevice, stream),
+                         "krylov_step: copy")) return rc;
+
+  std::vector<double> tdiag((size_t)2 * (max_dim + 2), 0.0), tsub((size_t)max_dim + 2, 0.0);
+  tdiag[0] = b0.real();
+  tdiag[1] = b0.imag();
+  std::vector<double> sc((size_t)max_dim + 2, 0.0);
+  sc[0] = s0;
+  std::vector<const void *> vec((size_t)max_dim + 2, nullptr);
+  vec[0] = v;
+  EventPair ev;
+  for (hipEvent_t &x : ev.e)
+    if (int rc = hip_check(hipEventCreateWithFlags(&x, hipEventDisableTiming), "krylov_step: event")) return rc;
+  int enqueued = 0, matvecs = 1;
+  auto enqueue = [&](int j) -> int {
+    double *rec = records + (size_t)j * kStepRow;
+    // iteration j reads w_j (row j + 1), orthogonalises it against basis
+    // vectors j - 1 and j and writes r_j = w_{j+1} into row j + 2
+    const void *pa = j > 0 ? (j == 1 ? v : row_ptr(j - 1)) : nullptr;
+    const void *pb = j == 0 ? v : row_ptr(j);
+    if (int rc = emu_sv_lanczos_iterate_dev(n, dtype, ham, sigma_re, sigma_im, row_ptr(j + 1), pa, pb,
+                                            rec + 16, row_ptr(j + 2), rec, rec + 5,
+                                            records + (size_t)(j + 1) * kStepRow + 16, rec + 24, workspace,
+                                            tile_log, min_contig, base_flags | EMU_SV_FLAG_KEEP_DIAG, stream))
+      return rc;
+    if (int rc = hip_check(hipMemcpyAsync(host_records + (size_t)j * kStepRow, rec, kStepRow * sizeof(double),
+                                          hipMemcpyDeviceToHost, stream), "krylov_step: copy"))
+      return rc;
+    if (int rc = hip_check(hipEventRecord(ev.e[j & 1], stream), "krylov_step: event")) return rc;
+    enqueued = j + 1;
+    ++matvecs;
+    return EMU_SV_OK;
+  };
+  std::vector<double> ex;
+  // exp(T) of the leading m x m block; with coefficients: the combination
+  auto texp = [&](int m, double n2, int want) -> int {
+    ex.assign(want ? 2 + 2 * (size_t)(m + 1) : 2, 0.0);
+    if (emu_sv_lanczos_exp(m, tdiag.data(), tsub.data(), sigma_re, sigma_im, n2, want, ex.data()) != EMU_SV_OK)
+      return fail(EMU_SV_EINVAL, "krylov_step: small exponential failed");
+    return EMU_SV_OK;
+  };
+  auto finish = [&](int m, double n2, int happy, int converged) -> int {
+    if (int rc = texp(m, n2, 1)) return rc;
+    const int count = happy ? m : m + 1;
+    std::vector<double> coef((size_t)2 * count);
+    for (int k = 0; k < count; ++k) {
+      coef[2 * k] = ex[2 + 2 * k] * sc[k] * initial_norm;
+      coef[2 * k + 1] = ex[3 + 2 * k] * sc[k] * initial_norm;
+    }
+    if (int rc = emu_sv_lincomb(n, dtype, count, vec.data(), coef.data(), out, stream)) return rc;
+    result_out[0] = m;
+    result_out[1] = converged;
+    result_out[2] = happy;
+    result_out[3] = matvecs;
+    return EMU_SV_OK;
+  };
+
+  if (int rc = enqueue(0)) return rc;
+  double last_err = -1.0;
+  for (int j = 0; j < max_dim; ++j) {
+    // speculate on iteration j+1 unless iteration j is likely to converge
+    // (half the last error estimate below the tolerance)
+    const bool likely_done = last_err >= 0.0 && 0.5 * last_err < exp_tolerance;
+    if (j + 1 < max_dim && !likely_done)
+      if (int rc = enqueue(j + 1)) return rc;
+    if (int rc = hip_check(hipEventSynchronize(ev.e[j & 1]), "krylov_step: wait")) return rc;
+    const double *rec = host_records + (size_t)j * kStepRow;
+    const double nrm = std::sqrt(std::max(ww, 0.0));  // ||op(v_j)||, before orthogonalisation
+    const double n2 = rec[24];
+    tsub[j] = n2;
+    if (n2 < norm_tolerance) return finish(j + 1, 0.0, 1, 1);  // happy breakdown
+    vec[j + 1] = row_ptr(j + 1);
+    sc[j + 1] = 1.0 / n2;
+    if (int rc = texp(j + 1, n2, 0)) return rc;
+    const double e1 = ex[0], err2 = ex[1] * nrm;
+    const double err = e1 < err2 ? e1 : e1 * err2 / (e1 - err2);
+    if (err < exp_tolerance) return finish(j + 1, n2, 0, 1);
+    last_err = err;
+    tdiag[2 * (j + 1)] = rec[25];
+    tdiag[2 * (j + 1) + 1] = rec[26];
+    ww = rec[29];
+    if (j + 1 < max_dim && enqueued == j + 1)
+      if (int rc = enqueue(j + 1)) return rc;  // the speculation was skipped and j did not converge
+  }
+  return finish(max_dim, tsub[max_dim - 1], 0, 0);
+}
+
 int emu_sv_apply_1q(int n, int dtype, void *psi, int bit, const double *a, hipStream_t stream) {
   if (int rc = check_n(n)) return rc;
   if (int rc = check_dtype(dtype)) return rc;

@@ -384,22 +384,57 @@ def _pipelined_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: fl
 PIPELINE = os.environ.get("EMU_SV_PIPELINE", "1") != "0"
 
 
+# Up to this many qubits a Hermitian Krylov step that needs only its result
+# runs in the library's driver (emu_sv_krylov_step: the loop of
+# _pipelined_lanczos in native code), which holds a basis of max_dim + 1
+# vectors for the call (101 x 64 MiB at 22 complex128 qubits).  Above it the
+# Python engine allocates the m + 2 vectors it uses; there an iteration's GPU
+# time (0.5 ms and more) already hides the interpreter's work.
+NATIVE_LANCZOS_MAX_QUBITS = 22
+_LDS_DEFAULT = autotune.Schedule("lds", 0, 0)
+
+
+def _lanczos_schedule(op: ScaledHamiltonian, v: torch.Tensor):
+    """(schedule, pipelined): the tuned H.psi schedule of this run and whether
+    the device-coefficient (pipelined) iteration applies to it."""
+    ham = op.hamiltonian
+    n = native.nqubits_of(v)
+    sched = autotune.best_schedule(n, v.dtype, v.device, ham.device_params(v.device), ham.complex)
+    if sched.kernel == "btile":  # one big tile: 2^15 (c64) / 2^14 (c128) amplitudes
+        tl = fl = 16 if v.dtype == torch.complex64 else 15
+    else:
+        tl = sched.tile_log or (13 if v.dtype == torch.complex64 else 12)
+        fl = sched.first_log or tl
+    ok = PIPELINE and sched.kernel in ("tile", "lds", "btile") and n >= max(tl, fl)
+    return sched, ok
+
+
+def _native_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
+                    norm_tolerance: float, max_krylov_dim: int, sched) -> KrylovExpResult:
+    """The Hermitian branch in one library call (emu_sv_krylov_step: the loop
+    of _pipelined_lanczos in native code)."""
+    shape = v.shape
+    v = v.contiguous().view(-1)
+    n = native.nqubits_of(v)
+    ham = op.hamiltonian
+    out = torch.empty_like(v)
+    KrylovStats.calls += 1
+    iters, converged, happy, matvecs = native.krylov_step(
+        n, ham.device_params(v.device), op.sigma, v, out, max_krylov_dim, exp_tolerance,
+        norm_tolerance, complex_coef=ham.complex, **sched.kwargs())
+    KrylovStats.matvecs += matvecs
+    return KrylovExpResult(out.view(shape), converged, happy, iters)
+
+
 def fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
                   norm_tolerance: float, max_krylov_dim: int,
                   rows: torch.Tensor | None = None) -> LanczosRun:
     """The fused engine: pipelined (device-side coefficients) when the tuned
     schedule is the tile or LDS kernel and N covers its tiles, else one
     read-back per iteration.  EMU_SV_PIPELINE=0 turns the pipeline off."""
-    ham = op.hamiltonian
-    n = native.nqubits_of(v)
-    if n == ham.nqubits:
-        sched = autotune.best_schedule(n, v.dtype, v.device, ham.device_params(v.device), ham.complex)
-        if sched.kernel == "btile":  # one big tile: 2^15 (c64) / 2^14 (c128) amplitudes
-            tl = fl = 16 if v.dtype == torch.complex64 else 15
-        else:
-            tl = sched.tile_log or (13 if v.dtype == torch.complex64 else 12)
-            fl = sched.first_log or tl
-        if PIPELINE and sched.kernel in ("tile", "lds", "btile") and n >= max(tl, fl):
+    if native.nqubits_of(v) == op.hamiltonian.nqubits:
+        sched, pipelined = _lanczos_schedule(op, v)
+        if pipelined:
             return _pipelined_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim, sched,
                                       rows)
     return _fused_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim, rows)
@@ -655,6 +690,14 @@ def krylov_exp_impl(op: Callable[[torch.Tensor], torch.Tensor], v: torch.Tensor,
             return _fused_arnoldi(op, v, exp_tolerance, norm_tolerance, max_krylov_dim)
         if getattr(op.hamiltonian, "noisy", False):
             raise ValueError("the noise term makes H non-Hermitian: use is_hermitian=False")
+        n = native.nqubits_of(v)
+        if PIPELINE and n == op.hamiltonian.nqubits and n <= NATIVE_LANCZOS_MAX_QUBITS \
+                and max_krylov_dim <= 256:
+            sched, pipelined = _lanczos_schedule(op, v)
+            # below the tuned sizes (or tiles larger than the register) the
+            # LDS kernel with its default tiles, one pass for small registers
+            return _native_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim,
+                                   sched if pipelined else _LDS_DEFAULT)
         run = fused_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim)
         return KrylovExpResult(run.combine(), run.converged, run.happy_breakdown,
                                run.iteration_count)

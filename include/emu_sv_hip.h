@@ -285,6 +285,34 @@ int emu_sv_arnoldi_orthonormalize(int n, int dtype, int count, const void *const
                                   const double *scales, void *w, double *records_out,
                                   void *workspace, hipStream_t stream);
 
+/* out = exp(sigma H) v: one Krylov time step of the Hermitian branch of
+ * krylov_exp (emu_base/math/krylov_exp.py:23-131 with is_hermitian=True, as
+ * called by emu_sv/time_evolution/torch.py:161-169) with its whole loop in
+ * the library: the opening H.psi with its dots, then per iteration
+ * emu_sv_lanczos_iterate_dev (the next iteration's coefficients formed on
+ * the device) and an asynchronous copy of its records; the host runs the
+ * reference's Expokit test one iteration behind (emu_sv_lanczos_exp on the
+ * tridiagonal T) and enqueues iteration j+1 before waiting for iteration j
+ * unless half the last error estimate is below the tolerance; the happy
+ * breakdown and the final combination of m+1 basis vectors (emu_sv_lincomb)
+ * follow the reference.  Waits for each iteration's records on the host;
+ * the final combination is left enqueued on stream.
+ *   v          device input state (2^n elements, not modified; out may alias it)
+ *   basis      device buffer of max_dim + 1 vectors of 2^n elements
+ *   records    device, emu_sv_krylov_step_records_doubles(max_dim) doubles
+ *   host_records  pinned host buffer of the same size
+ *   flags      kernel kind (tile, btile or LDS; n must cover its tiles),
+ *              EMU_SV_FLAG_COMPLEX
+ *   result_out host int[4]: iterations, converged, happy breakdown, matvecs
+ * max_dim in [1, 256].  Returns EMU_SV_OK also when the iteration did not
+ * converge (result_out[1] == 0: the reference's RecursionError). */
+size_t emu_sv_krylov_step_records_doubles(int max_dim);
+int emu_sv_krylov_step(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
+                       const void *v, void *out, void *basis, int max_dim, double exp_tolerance,
+                       double norm_tolerance, double *records, double *host_records,
+                       void *workspace, int tile_log, int min_contig, int flags,
+                       hipStream_t stream, int *result_out);
+
 /* The Arnoldi Gram-Schmidt step in its Gram-matrix form, for the pipelined
  * engines on vectors outside the one-workgroup range.  Replaces the same
  * loop as emu_sv_arnoldi_orthogonalize (emu_base/math/krylov_exp.py:54-61

@@ -27,19 +27,28 @@ def _random_register(n, steps, seed, with_phase):
 
 
 def _sequential(reg, dtype, tol):
-    """The single-register fused engine, step by step."""
+    """The single-register fused engine, step by step, with one read-back per
+    iteration (no speculative iterations, so its matvec count is the
+    algorithm's: m + 1 per step, as the batch engine's)."""
+    import importlib
+
     from emulators_amd.emu_base.math.krylov_exp import KrylovStats, ScaledHamiltonian, krylov_exp
     from emulators_amd.emu_sv import RydbergHamiltonian
 
+    kx = importlib.import_module("emulators_amd.emu_base.math.krylov_exp")
     n = reg.nqubits
     psi = torch.zeros(2**n, dtype=dtype, device="cuda")
     psi[0] = 1.0
     KrylovStats.reset()
-    for k in range(reg.steps):
-        ham = RydbergHamiltonian(torch.as_tensor(reg.omegas[k]), torch.as_tensor(reg.deltas[k]),
-                                 torch.as_tensor(reg.phis[k]), torch.as_tensor(reg.interaction_matrix),
-                                 device="cuda")
-        psi = krylov_exp(ScaledHamiltonian(ham, -1j * float(reg.dts[k])), psi, tol, tol)
+    saved, kx.PIPELINE = kx.PIPELINE, False
+    try:
+        for k in range(reg.steps):
+            ham = RydbergHamiltonian(torch.as_tensor(reg.omegas[k]), torch.as_tensor(reg.deltas[k]),
+                                     torch.as_tensor(reg.phis[k]), torch.as_tensor(reg.interaction_matrix),
+                                     device="cuda")
+            psi = krylov_exp(ScaledHamiltonian(ham, -1j * float(reg.dts[k])), psi, tol, tol)
+    finally:
+        kx.PIPELINE = saved
     return psi, KrylovStats.matvecs
 
 

@@ -59,6 +59,47 @@ def test_iteration_count_matches_oracle(gpu_device, n, with_phase):
     assert torch.allclose(got.result.cpu(), ref.result, atol=1e-12, rtol=0)
 
 
+@pytest.mark.parametrize("n,dtype,with_phase", [(3, torch.complex128, False), (8, torch.complex128, True),
+                                                 (12, torch.complex64, False), (16, torch.complex128, True),
+                                                 (16, torch.complex64, False), (19, torch.complex128, False),
+                                                 (21, torch.complex64, True)])
+def test_native_step_matches_python_engines(gpu_device, monkeypatch, n, dtype, with_phase):
+    """emu_sv_krylov_step (the Lanczos loop in the library) against the Python
+    engines on the same schedule: the pipelined engine where it applies, else
+    the one-read-back engine; the same iteration count and the same result to
+    rounding (1e-12 relative in complex128, 1e-5 in complex64); and the
+    oracle's iteration count and result (complex128, n <= 16)."""
+    import importlib
+
+    from emulators_amd.emu_sv import RydbergHamiltonian
+    kx = importlib.import_module("emulators_amd.emu_base.math.krylov_exp")
+    print(f"native krylov step n={n} {dtype} phase={with_phase}", flush=True)
+    om, de, ph, u = _randn_params(n, with_phase, 400 + n)
+    om, de, u = om * 8, de * 4, u * 15
+    psi = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(n))
+    dt, tol = 0.04, 1e-10
+    ham = RydbergHamiltonian(om, de, ph, u, device=gpu_device)
+    op = kx.ScaledHamiltonian(ham, -1j * dt)
+    v = psi.to(gpu_device, dtype)
+    calls = []
+    real = kx._native_lanczos
+    monkeypatch.setattr(kx, "_native_lanczos", lambda *a, **k: calls.append(1) or real(*a, **k))
+    got = kx.krylov_exp_impl(op, v.clone(), True, tol, tol)
+    assert calls, "krylov_exp_impl did not take the library's step driver"
+    run = kx.fused_lanczos(op, v.clone(), tol, tol, kx.DEFAULT_MAX_KRYLOV_DIM)
+    ref = run.combine()
+    assert got.converged and run.converged
+    assert got.iteration_count == run.iteration_count
+    scale = float(ref.abs().max())
+    diff = float((got.result - ref).abs().max())
+    assert diff <= (1e-12 if dtype == torch.complex128 else 1e-5) * scale, diff
+    if dtype == torch.complex128 and n <= 16:
+        ham_o = orc.OracleHamiltonian(om, de, ph, u)
+        ora = orc.krylov_exp_impl(lambda x: -1j * dt * (ham_o * x), psi.clone(), True, tol, tol)
+        assert got.iteration_count == ora.iteration_count
+        assert float((got.result.cpu() - ora.result).abs().max()) < 1e-11
+
+
 def test_happy_breakdown_and_nonconvergence(gpu_device):
     from emulators_amd.emu_base.math.krylov_exp import ScaledHamiltonian, krylov_exp, krylov_exp_impl
     from emulators_amd.emu_sv import RydbergHamiltonian
@@ -301,12 +342,14 @@ def test_pipelined_engine_matches_unpipelined(gpu_device, monkeypatch, dtype, n,
     ref = orc.krylov_exp(lambda x: -0.01j * (orc.OracleHamiltonian(om, de, ph, u) * x), psi.clone(),
                          tol, tol)
     assert float((a - ref).norm()) < (1e-9 if dtype == torch.complex128 else 1e-5)
-    # and through krylov_exp (the dispatcher picks the pipelined engine)
+    # and through krylov_exp: at these sizes the dispatcher takes the
+    # library's step driver (emu_sv_krylov_step, the same pipelined loop in
+    # native code) on the same schedule
     calls = []
-    real = kx._pipelined_lanczos
-    monkeypatch.setattr(kx, "_pipelined_lanczos", lambda *a, **k: calls.append(1) or real(*a, **k))
+    real = kx._native_lanczos
+    monkeypatch.setattr(kx, "_native_lanczos", lambda *a, **k: calls.append(a[-1]) or real(*a, **k))
     out = kx.krylov_exp(op, v.clone(), tol, tol)
-    assert calls, "krylov_exp did not take the pipelined engine"
+    assert calls == [sched], "krylov_exp did not take the library's step driver on the tuned schedule"
     assert float((out.cpu().to(torch.complex128) - a).norm()) < tolv
 
 
