@@ -74,8 +74,9 @@ SIGNATURES = {
     "emu_sv_arnoldi_orthonormalize": (_i, [_i, _i, _i, ctypes.POINTER(_vp), _dp, _vp, _vp, _vp,
                                            _vp]),
     "emu_sv_krylov_step_records_doubles": (ctypes.c_size_t, [_i]),
+    "emu_sv_krylov_step_decomp_doubles": (ctypes.c_size_t, [_i]),
     "emu_sv_krylov_step": (_i, [_i, _i, _vp, _d, _d, _vp, _vp, _vp, _i, _d, _d, _vp, _vp, _vp, _i, _i,
-                                _i, _vp, ctypes.POINTER(_i)]),
+                                _i, _vp, ctypes.POINTER(_i), _vp]),
     "emu_sv_arnoldi_gram": (_i, [_i, _i, _i, ctypes.POINTER(_vp), _vp, _d, _vp, _i, _vp, _vp]),
     "emu_sv_arnoldi_gram_state_doubles": (ctypes.c_size_t, [_i]),
     "emu_sv_arnoldi_exp_records_doubles": (ctypes.c_size_t, [_i]),
@@ -572,28 +573,38 @@ class _KrylovStepBuffers:
 def krylov_step(n: int, ham_dev: torch.Tensor, sigma: complex, v: torch.Tensor, out: torch.Tensor,
                 max_dim: int, exp_tolerance: float, norm_tolerance: float, complex_coef: bool = True,
                 tile_log: int = 0, min_contig: int = 0, kernel: str = "lds", partners: int = 0,
-                first_log: int = 0) -> tuple[int, bool, bool, int]:
+                first_log: int = 0, basis: torch.Tensor | None = None, decomp: bool = False):
     """out = exp(sigma H) v by the library's pipelined Lanczos driver
     (emu_sv_krylov_step); returns (iterations, converged, happy breakdown,
-    matvecs).  The basis (max_dim + 1 vectors) comes from torch's allocator
-    for the call."""
+    matvecs, decomposition or None).  ``basis``: a contiguous device buffer of
+    at least (max_dim + 1) * 2**n elements that receives basis rows 1..
+    (default: one from torch's allocator for the call).  ``decomp``: also
+    return the Lanczos decomposition (numpy, the layout of
+    emu_sv_krylov_step_decomp_doubles)."""
     _require_device_vec(v, "krylov_step")
     _require_device_vec(out, "krylov_step")
     if v.numel() != 1 << n or out.numel() != 1 << n or out.dtype != v.dtype:
         raise ValueError("krylov_step: v and out must be 2**n elements of one dtype")
     _require_doubles(ham_dev, EMU_SV_HAM_DOUBLES, "krylov_step (parameter block)")
     bufs = _KrylovStepBuffers.get(v.device, max_dim)
-    basis = torch.empty((max_dim + 1) << n, dtype=v.dtype, device=v.device)
+    if basis is None:
+        basis = torch.empty((max_dim + 1) << n, dtype=v.dtype, device=v.device)
+    elif not (basis.is_cuda and basis.device == v.device and basis.dtype == v.dtype
+              and basis.is_contiguous() and basis.numel() >= (max_dim + 1) << n):
+        raise ValueError("krylov_step: basis must be a contiguous device buffer of "
+                         "(max_dim + 1) * 2**n elements of v's dtype")
     ws = Workspace.get(v.device, n)
     flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags(kernel, partners, first_log)
     res = (ctypes.c_int * 4)()
+    dec = np.zeros(int(LIB.emu_sv_krylov_step_decomp_doubles(max_dim))) if decomp else None
     check(LIB.emu_sv_krylov_step(n, dtype_code(v.dtype), _ptr(ham_dev), float(sigma.real),
                                  float(sigma.imag), _ptr(v), _ptr(out), _ptr(basis), max_dim,
                                  float(exp_tolerance), float(norm_tolerance), _ptr(bufs.records),
                                  _ptr(bufs.host), _ptr(ws.scratch), tile_log, min_contig, flags,
-                                 _stream(v.device), res), "krylov_step")
-    del basis
-    return int(res[0]), bool(res[1]), bool(res[2]), int(res[3])
+                                 _stream(v.device), res,
+                                 dec.ctypes.data_as(ctypes.c_void_p) if dec is not None else None),
+          "krylov_step")
+    return int(res[0]), bool(res[1]), bool(res[2]), int(res[3]), dec
 
 
 def apply_1q(n: int, psi: torch.Tensor, qubit: int, op) -> torch.Tensor:

@@ -3658,11 +3658,16 @@ size_t emu_sv_krylov_step_records_doubles(int max_dim) {
   return (size_t)kStepRow * (max_dim + 3);
 }
 
+size_t emu_sv_krylov_step_decomp_doubles(int max_dim) {
+  if (max_dim < 1 || max_dim > kArnoldiMaxDim) return 0;
+  return 4 + 8 * (size_t)(max_dim + 2);
+}
+
 int emu_sv_krylov_step(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
                        const void *v, void *out, void *basis, int max_dim, double exp_tolerance,
                        double norm_tolerance, double *records, double *host_records,
                        void *workspace, int tile_log, int min_contig, int flags,
-                       hipStream_t stream, int *result_out) {
+                       hipStream_t stream, int *result_out, double *decomp_out) {
   if (int rc = check_n(n)) return rc;
   if (int rc = check_dtype(dtype)) return rc;
   if (!ham || !v || !out || !basis || !records || !host_records || !workspace || !result_out)
@@ -3716,6 +3721,7 @@ int emu_sv_krylov_step(int n, int dtype, const double *ham, double sigma_re, dou
                          "krylov_step: copy")) return rc;
 
   std::vector<double> tdiag((size_t)2 * (max_dim + 2), 0.0), tsub((size_t)max_dim + 2, 0.0);
+  std::vector<double> tsup((size_t)2 * (max_dim + 2), 0.0);  // T[k, k+1]
   tdiag[0] = b0.real();
   tdiag[1] = b0.imag();
   std::vector<double> sc((size_t)max_dim + 2, 0.0);
@@ -3766,6 +3772,19 @@ int emu_sv_krylov_step(int n, int dtype, const double *ham, double sigma_re, dou
     result_out[1] = converged;
     result_out[2] = happy;
     result_out[3] = matvecs;
+    if (decomp_out) {  // the Lanczos decomposition (layout: emu_sv_krylov_step_decomp_doubles)
+      const size_t M = (size_t)max_dim + 2;
+      double *d = decomp_out;
+      d[0] = m;
+      d[1] = happy ? 0.0 : n2;
+      d[2] = happy;
+      d[3] = converged;
+      for (int k = 0; k < 2 * m; ++k) d[4 + k] = tdiag[k];
+      for (int k = 0; k < m; ++k) d[4 + 2 * M + k] = tsub[k];
+      for (int k = 0; k < 2 * m; ++k) d[4 + 3 * M + k] = tsup[k];
+      for (int k = 0; k < count; ++k) d[4 + 5 * M + k] = sc[k];
+      for (int k = 0; k < 2 * count; ++k) d[4 + 6 * M + k] = ex[2 + k];
+    }
     return EMU_SV_OK;
   };
 
@@ -3792,6 +3811,8 @@ int emu_sv_krylov_step(int n, int dtype, const double *ham, double sigma_re, dou
     last_err = err;
     tdiag[2 * (j + 1)] = rec[25];
     tdiag[2 * (j + 1) + 1] = rec[26];
+    tsup[2 * j] = rec[27];
+    tsup[2 * j + 1] = rec[28];
     ww = rec[29];
     if (j + 1 < max_dim && enqueued == j + 1)
       if (int rc = enqueue(j + 1)) return rc;  // the speculation was skipped and j did not converge

@@ -122,6 +122,7 @@ class LanczosRun:
     happy_breakdown: bool
     iteration_count: int
     n2: float
+    result: torch.Tensor | None = None  # exp(op) v when the engine formed it already
 
     @property
     def m(self) -> int:
@@ -141,6 +142,10 @@ class LanczosRun:
 
     def combine(self, coeffs: np.ndarray | None = None, out: torch.Tensor | None = None) -> torch.Tensor:
         """initial_norm * sum_k coeffs_k V_k (default coeffs: exp(T) e_0)."""
+        if coeffs is None and self.result is not None:
+            if out is None:
+                return self.result
+            return out.copy_(self.result)
         c = self.coeffs if coeffs is None else np.asarray(coeffs)
         k = len(c)
         n = native.nqubits_of(self.basis[0])
@@ -419,11 +424,60 @@ def _native_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float
     ham = op.hamiltonian
     out = torch.empty_like(v)
     KrylovStats.calls += 1
-    iters, converged, happy, matvecs = native.krylov_step(
+    iters, converged, happy, matvecs, _ = native.krylov_step(
         n, ham.device_params(v.device), op.sigma, v, out, max_krylov_dim, exp_tolerance,
         norm_tolerance, complex_coef=ham.complex, **sched.kwargs())
     KrylovStats.matvecs += matvecs
     return KrylovExpResult(out.view(shape), converged, happy, iters)
+
+
+def _native_lanczos_run(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
+                        norm_tolerance: float, max_krylov_dim: int, sched,
+                        rows: torch.Tensor | None = None) -> LanczosRun:
+    """The Lanczos decomposition of one step by the library's driver
+    (emu_sv_krylov_step with its decomposition output), for callers that
+    reuse the Krylov space (the adjoint).  With ``rows`` (a contiguous
+    (max_krylov_dim + 2, 2**n) tensor) the basis lands in its rows, row 0 a
+    copy of v; otherwise the used rows are kept in a compact tensor.  The
+    combination exp(op) v is computed by the driver and cached on the run."""
+    ham = op.hamiltonian
+    v = v.contiguous().view(-1)
+    n = native.nqubits_of(v)
+    KrylovStats.calls += 1
+    if rows is not None:
+        if rows.dim() != 2 or rows.shape[0] < max_krylov_dim + 2 or rows.shape[1] != v.numel() \
+                or rows.dtype != v.dtype or not rows.is_contiguous():
+            raise ValueError("rows must be a contiguous (max_krylov_dim + 2, 2**n) tensor of v's dtype")
+        rows[0].copy_(v)
+        v = rows[0]
+        buf = rows[1:]
+    else:
+        buf = torch.empty((max_krylov_dim + 1, v.numel()), dtype=v.dtype, device=v.device)
+    out = torch.empty_like(v)
+    iters, converged, happy, matvecs, dec = native.krylov_step(
+        n, ham.device_params(v.device), op.sigma, v, out, max_krylov_dim, exp_tolerance,
+        norm_tolerance, complex_coef=ham.complex, basis=buf, decomp=True, **sched.kwargs())
+    KrylovStats.matvecs += matvecs
+    big = max_krylov_dim + 2
+    m = int(dec[0])
+    count = m if happy else m + 1
+    T = np.zeros((m, m), dtype=np.complex128)
+    diag = dec[4: 4 + 2 * m].view(np.complex128)
+    sub = dec[4 + 2 * big: 4 + 2 * big + m]
+    sup = dec[4 + 3 * big: 4 + 3 * big + 2 * m].view(np.complex128)
+    idx = np.arange(m)
+    T[idx, idx] = diag
+    T[idx[1:], idx[:-1]] = sub[: m - 1]
+    T[idx[:-1], idx[1:]] = sup[: m - 1]
+    scales = [float(x) for x in dec[4 + 5 * big: 4 + 5 * big + count]]
+    coeffs = dec[4 + 6 * big: 4 + 6 * big + 2 * count].view(np.complex128).copy()
+    if rows is None and count > 1:
+        buf = buf[: count - 1].clone()  # only the rows this run used stay alive
+    basis = [v] + [buf[k - 1] for k in range(1, count)]
+    run = LanczosRun(basis, scales, T, coeffs, 1.0 / scales[0], converged, happy, m,
+                     float(dec[1]))
+    run.result = out
+    return run
 
 
 def fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
@@ -432,8 +486,12 @@ def fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
     """The fused engine: pipelined (device-side coefficients) when the tuned
     schedule is the tile or LDS kernel and N covers its tiles, else one
     read-back per iteration.  EMU_SV_PIPELINE=0 turns the pipeline off."""
-    if native.nqubits_of(v) == op.hamiltonian.nqubits:
+    n = native.nqubits_of(v)
+    if n == op.hamiltonian.nqubits:
         sched, pipelined = _lanczos_schedule(op, v)
+        if PIPELINE and n <= NATIVE_LANCZOS_MAX_QUBITS and max_krylov_dim <= 256:
+            return _native_lanczos_run(op, v, exp_tolerance, norm_tolerance, max_krylov_dim,
+                                       sched if pipelined else _LDS_DEFAULT, rows)
         if pipelined:
             return _pipelined_lanczos(op, v, exp_tolerance, norm_tolerance, max_krylov_dim, sched,
                                       rows)

@@ -304,14 +304,25 @@ int emu_sv_arnoldi_orthonormalize(int n, int dtype, int count, const void *const
  *   flags      kernel kind (tile, btile or LDS; n must cover its tiles),
  *              EMU_SV_FLAG_COMPLEX
  *   result_out host int[4]: iterations, converged, happy breakdown, matvecs
+ *   decomp_out host, emu_sv_krylov_step_decomp_doubles(max_dim) doubles, or
+ *              NULL: the Lanczos decomposition for callers that reuse the
+ *              Krylov space (the adjoint).  With M = max_dim + 2 and m the
+ *              iterations: [0] m, [1] T[m, m-1] (0 after a happy breakdown),
+ *              [2] happy, [3] converged; T[k, k] (complex) at 4 + 2k;
+ *              T[k+1, k] (real) at 4 + 2M + k; T[k, k+1] (complex) at
+ *              4 + 3M + 2k; the basis scales (basis vector k is scale_k times
+ *              v for k = 0, basis row k for k >= 1) at 4 + 5M + k; the
+ *              combination coefficients exp(T_ext) e_0 (complex, m + 1 of
+ *              them, m after a breakdown) at 4 + 6M + 2k.
  * max_dim in [1, 256].  Returns EMU_SV_OK also when the iteration did not
  * converge (result_out[1] == 0: the reference's RecursionError). */
 size_t emu_sv_krylov_step_records_doubles(int max_dim);
+size_t emu_sv_krylov_step_decomp_doubles(int max_dim);
 int emu_sv_krylov_step(int n, int dtype, const double *ham, double sigma_re, double sigma_im,
                        const void *v, void *out, void *basis, int max_dim, double exp_tolerance,
                        double norm_tolerance, double *records, double *host_records,
                        void *workspace, int tile_log, int min_contig, int flags,
-                       hipStream_t stream, int *result_out);
+                       hipStream_t stream, int *result_out, double *decomp_out);
 
 /* The Arnoldi Gram-Schmidt step in its Gram-matrix form, for the pipelined
  * engines on vectors outside the one-workgroup range.  Replaces the same
