@@ -100,6 +100,49 @@ def test_native_step_matches_python_engines(gpu_device, monkeypatch, n, dtype, w
         assert float((got.result.cpu() - ora.result).abs().max()) < 1e-11
 
 
+@pytest.mark.parametrize("n,dtype", [(9, torch.complex128), (16, torch.complex128), (17, torch.complex64)])
+@pytest.mark.parametrize("with_rows", [False, True])
+def test_native_decomposition_matches_pipelined(gpu_device, n, dtype, with_rows):
+    """The Lanczos decomposition returned by emu_sv_krylov_step (the adjoint's
+    source of Krylov spaces) against the Python pipelined engine's on the same
+    schedule: T, the basis scales, the combination coefficients and every
+    basis vector, to rounding; with ``rows`` the basis lands in the caller's
+    rows (row 0 a copy of v)."""
+    import importlib
+
+    from emulators_amd.emu_sv import RydbergHamiltonian
+    kx = importlib.import_module("emulators_amd.emu_base.math.krylov_exp")
+    om, de, ph, u = _randn_params(n, True, 700 + n)
+    om, de, u = om * 6, de * 3, u * 10
+    psi = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(n))
+    dt, tol, mdim = 0.05, 1e-10, 40
+    ham = RydbergHamiltonian(om, de, ph, u, device=gpu_device)
+    op = kx.ScaledHamiltonian(ham, -1j * dt)
+    v = psi.to(gpu_device, dtype)
+    sched, pipelined = kx._lanczos_schedule(op, v)
+    if not pipelined:
+        sched = kx._LDS_DEFAULT
+    rows = torch.empty((mdim + 2, 2**n), dtype=dtype, device=gpu_device) if with_rows else None
+    nat = kx._native_lanczos_run(op, v.clone(), tol, tol, mdim, sched, rows)
+    ref = kx._pipelined_lanczos(op, v.clone(), tol, tol, mdim, sched)
+    assert nat.converged and ref.converged and not nat.happy_breakdown
+    assert nat.iteration_count == ref.iteration_count and nat.m == ref.m
+    rel = 1e-12 if dtype == torch.complex128 else 1e-5
+    assert np.abs(nat.T - ref.T).max() <= rel * np.abs(ref.T).max()
+    assert np.allclose(nat.scales, ref.scales, rtol=rel, atol=0)
+    assert np.abs(nat.coeffs - ref.coeffs).max() <= rel * np.abs(ref.coeffs).max()
+    assert abs(nat.n2 - ref.n2) <= rel * ref.n2
+    assert len(nat.basis) == len(ref.basis)
+    for a, b in zip(nat.basis, ref.basis):
+        assert float((a - b).abs().max()) <= rel * float(b.abs().max())
+    if with_rows:
+        assert nat.basis[0].data_ptr() == rows[0].data_ptr()
+        assert nat.basis[1].data_ptr() == rows[1].data_ptr()
+    # the cached combination equals a fresh one from the decomposition
+    fresh = nat.combine(nat.coeffs)
+    assert float((nat.combine() - fresh).abs().max()) <= rel * float(fresh.abs().max())
+
+
 def test_happy_breakdown_and_nonconvergence(gpu_device):
     from emulators_amd.emu_base.math.krylov_exp import ScaledHamiltonian, krylov_exp, krylov_exp_impl
     from emulators_amd.emu_sv import RydbergHamiltonian
