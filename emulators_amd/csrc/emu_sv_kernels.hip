@@ -2205,8 +2205,9 @@ struct OrthoIn {
 // leaves rows whose stamp matches and rebuilds the others, so a table is
 // never reused for a different block, size or tile, whatever address the
 // block has.  Without keep every row is rebuilt.
-// Set by emu_sv_arnoldi_exp around its matvecs after the first: the table
-// was built by the first one, on this workspace and stream, with nothing in
+// Set by emu_sv_arnoldi_exp around its matvecs after the first, and by
+// emu_sv_krylov_step around its iterations: the table was built by the
+// step's first matvec, on this workspace and stream, with nothing in
 // between, so the stamp check (a launch on the iteration's critical path)
 // is skipped.  Calling-thread state, never set across a return to the caller.
 thread_local bool t_diag_known_fresh = false;
@@ -3497,6 +3498,17 @@ int emu_sv_arnoldi_orthonormalize(int n, int dtype, int count, const void *const
 // is already below the tolerance), so the host's small exponential and test
 // overlap the device's next matvec.  No per-iteration interpreter work.
 namespace {
+// Wait for an event by polling it: the library's drivers wait once per
+// Krylov iteration for a few tens of microseconds of GPU work, where a
+// blocking wait's wake-up costs more than the poll.
+int wait_event(hipEvent_t e, const char *what) {
+  for (;;) {
+    const hipError_t q = hipEventQuery(e);
+    if (q == hipSuccess) return EMU_SV_OK;
+    if (q != hipErrorNotReady) return hip_check(q, what);
+  }
+}
+
 struct EventPair {
   hipEvent_t e[2] = {nullptr, nullptr};
   ~EventPair() {
@@ -3615,7 +3627,7 @@ int emu_sv_arnoldi_exp(int n, int dtype, const double *ham, double sigma_re, dou
     const bool likely_done = last_err >= 0.0 && 0.5 * last_err < exp_tolerance;
     if (j + 1 < max_dim && !likely_done)
       if (int rc = enqueue(j + 1)) return rc;
-    if (int rc = hip_check(hipEventSynchronize(ev.e[j & 1]), "arnoldi_exp: wait")) return rc;
+    if (int rc = wait_event(ev.e[j & 1], "arnoldi_exp: wait")) return rc;
     const double *rec = host_records + (size_t)j * row;
     const double nrm = std::sqrt(std::max(rec[0], 0.0));  // ||op(v_j)||, before orthogonalisation
     // Gram form: the records hold T[k, j] itself, and the next (unnormalised)
@@ -3738,11 +3750,16 @@ int emu_sv_krylov_step(int n, int dtype, const double *ham, double sigma_re, dou
     // vectors j - 1 and j and writes r_j = w_{j+1} into row j + 2
     const void *pa = j > 0 ? (j == 1 ? v : row_ptr(j - 1)) : nullptr;
     const void *pb = j == 0 ? v : row_ptr(j);
-    if (int rc = emu_sv_lanczos_iterate_dev(n, dtype, ham, sigma_re, sigma_im, row_ptr(j + 1), pa, pb,
-                                            rec + 16, row_ptr(j + 2), rec, rec + 5,
-                                            records + (size_t)(j + 1) * kStepRow + 16, rec + 24, workspace,
-                                            tile_log, min_contig, base_flags | EMU_SV_FLAG_KEEP_DIAG, stream))
-      return rc;
+    // the opening H.psi built the first-pass diagonal table on this
+    // workspace and stream: the iterations skip its stamp check
+    t_diag_known_fresh = true;
+    const int irc = emu_sv_lanczos_iterate_dev(n, dtype, ham, sigma_re, sigma_im, row_ptr(j + 1), pa, pb,
+                                               rec + 16, row_ptr(j + 2), rec, rec + 5,
+                                               records + (size_t)(j + 1) * kStepRow + 16, rec + 24,
+                                               workspace, tile_log, min_contig,
+                                               base_flags | EMU_SV_FLAG_KEEP_DIAG, stream);
+    t_diag_known_fresh = false;
+    if (irc) return irc;
     if (int rc = hip_check(hipMemcpyAsync(host_records + (size_t)j * kStepRow, rec, kStepRow * sizeof(double),
                                           hipMemcpyDeviceToHost, stream), "krylov_step: copy"))
       return rc;
@@ -3796,7 +3813,7 @@ int emu_sv_krylov_step(int n, int dtype, const double *ham, double sigma_re, dou
     const bool likely_done = last_err >= 0.0 && 0.5 * last_err < exp_tolerance;
     if (j + 1 < max_dim && !likely_done)
       if (int rc = enqueue(j + 1)) return rc;
-    if (int rc = hip_check(hipEventSynchronize(ev.e[j & 1]), "krylov_step: wait")) return rc;
+    if (int rc = wait_event(ev.e[j & 1], "krylov_step: wait")) return rc;
     const double *rec = host_records + (size_t)j * kStepRow;
     const double nrm = std::sqrt(std::max(ww, 0.0));  // ||op(v_j)||, before orthogonalisation
     const double n2 = rec[24];
