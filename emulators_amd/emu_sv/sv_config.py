@@ -15,9 +15,12 @@ Lanczos coefficients, reductions and the small exponential stay FP64).
 ``to_abstract_repr`` / ``from_abstract_repr`` serialise a config as JSON
 (sv_config.py and pulser's EmulationConfig): the keyword arguments, the
 observables (type, tag suffix, evaluation times and their own options) and
-an initial state as eigenstates plus {bitstring: amplitude}.  Complex
-amplitudes are written as [re, im] pairs; pulser's own schema is not
-available here, so the wire format matches this package's reader only.
+an initial state as eigenstates plus {bitstring: amplitude}.  A real
+amplitude is written as a plain number, so the parsed JSON compares equal
+to the amplitudes given (test/emu_sv/test_svconfig.py:76-98); a complex
+one as {"real": re, "imag": im}.  The reader also takes [re, im] pairs.
+pulser's own schema is not available here, so the rest of the wire format
+matches this package's reader only.
 """
 
 from __future__ import annotations
@@ -143,12 +146,25 @@ def _state_repr(state) -> dict:
         amps = {format(i, f"0{n}b").replace("1", "r").replace("0", "g"): complex(v)
                 for i, v in enumerate(data.tolist()) if v != 0}
     return {"eigenstates": list(state.eigenstates),
-            "amplitudes": {k: [complex(v).real, complex(v).imag] for k, v in amps.items()}}
+            "amplitudes": {k: _amplitude_repr(v) for k, v in amps.items()}}
+
+
+def _amplitude_repr(v):
+    v = complex(v)
+    return v.real if v.imag == 0.0 else {"real": v.real, "imag": v.imag}
+
+
+def _amplitude_from_repr(v) -> complex:
+    if isinstance(v, dict):
+        return complex(v["real"], v["imag"])
+    if isinstance(v, (list, tuple)):
+        return complex(v[0], v[1])
+    return complex(v)
 
 
 def _state_from_repr(d: dict):
     from emulators_amd.emu_sv.state_vector import StateVector
-    amps = {k: complex(v[0], v[1]) for k, v in d["amplitudes"].items()}
+    amps = {k: _amplitude_from_repr(v) for k, v in d["amplitudes"].items()}
     return StateVector.from_state_amplitudes(eigenstates=tuple(d["eigenstates"]), amplitudes=amps)
 
 

@@ -49,6 +49,21 @@ def test_serialization_with_state():
     text = SVConfig(initial_state=state).to_abstract_repr()
     d = json.loads(text)
     assert d["initial_state"]["eigenstates"] == basis
-    assert {k: complex(*v) for k, v in d["initial_state"]["amplitudes"].items()} == amp
+    assert d["initial_state"]["amplitudes"] == amp  # as the reference compares them
     back = SVConfig.from_abstract_repr(text)
     assert back.initial_state.data.cpu().allclose(state.data.cpu())
+
+
+def test_serialization_with_complex_state():
+    """Complex amplitudes as {"real", "imag"}; the reader also takes [re, im]."""
+    basis = ["r", "g"]
+    amp = {"gg": 0.6 + 0.0j, "gr": 0.0 + 0.8j}
+    state = StateVector.from_state_amplitudes(eigenstates=basis, amplitudes=amp)
+    text = SVConfig(initial_state=state).to_abstract_repr()
+    d = json.loads(text)
+    assert d["initial_state"]["amplitudes"] == {"gg": 0.6, "gr": {"real": 0.0, "imag": 0.8}}
+    back = SVConfig.from_abstract_repr(text)
+    assert back.initial_state.data.cpu().allclose(state.data.cpu())
+    d["initial_state"]["amplitudes"] = {"gg": [0.6, 0.0], "gr": [0.0, 0.8]}
+    again = SVConfig.from_abstract_repr(json.dumps(d))
+    assert again.initial_state.data.cpu().allclose(state.data.cpu())
