@@ -502,25 +502,24 @@ def arnoldi_gram(n: int, basis: list[torch.Tensor], w: torch.Tensor, v0_scale: f
 
 
 class _ArnoldiExpBuffers:
-    """Per-(device, stream, n, dtype, max_dim) buffers of emu_sv_arnoldi_exp:
-    the basis, the device records and their pinned host mirror."""
+    """Per-(device, stream, max_dim) device records of emu_sv_arnoldi_exp and
+    their pinned host mirror (the basis is allocated per call)."""
 
     _cache: dict = {}
 
-    def __init__(self, device, n: int, dtype, max_dim: int):
+    def __init__(self, device, max_dim: int):
         rows = int(LIB.emu_sv_arnoldi_exp_records_doubles(max_dim))
         if rows <= 0:
             raise ValueError("arnoldi_exp: max_dim must be in [1, 256]")
-        self.basis = torch.empty(max_dim << n, dtype=dtype, device=device)
         self.records = torch.zeros(rows, dtype=torch.float64, device=device)
         self.host = torch.zeros(rows, dtype=torch.float64, pin_memory=True)
 
     @classmethod
-    def get(cls, device, n: int, dtype, max_dim: int) -> "_ArnoldiExpBuffers":
-        key = (stream_key(device), n, dtype, max_dim)
+    def get(cls, device, max_dim: int) -> "_ArnoldiExpBuffers":
+        key = (stream_key(device), max_dim)
         b = cls._cache.get(key)
         if b is None:
-            b = cls._cache[key] = cls(device, n, dtype, max_dim)
+            b = cls._cache[key] = cls(device, max_dim)
         return b
 
 
@@ -534,14 +533,15 @@ def arnoldi_exp(n: int, ham_dev: torch.Tensor, sigma: complex, v: torch.Tensor, 
     _require_device_vec(out, "arnoldi_exp")
     if v.numel() != 1 << n or out.numel() != 1 << n or out.dtype != v.dtype:
         raise ValueError("arnoldi_exp: v and out must be 2**n elements of one dtype")
-    bufs = _ArnoldiExpBuffers.get(v.device, n, v.dtype, max_dim)
+    bufs = _ArnoldiExpBuffers.get(v.device, max_dim)
+    basis = torch.empty(max_dim << n, dtype=v.dtype, device=v.device)
     ws = Workspace.get(v.device, n)
     flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags(kernel, partners, first_log)
     if noise:
         flags |= EMU_SV_FLAG_NOISE
     res = (ctypes.c_int * 4)()
     check(LIB.emu_sv_arnoldi_exp(n, dtype_code(v.dtype), _ptr(ham_dev), float(sigma.real), float(sigma.imag),
-                                 _ptr(v), _ptr(out), _ptr(bufs.basis), max_dim, float(exp_tolerance),
+                                 _ptr(v), _ptr(out), _ptr(basis), max_dim, float(exp_tolerance),
                                  float(norm_tolerance), _ptr(bufs.records), _ptr(bufs.host),
                                  _ptr(ws.scratch), tile_log, min_contig, flags, _stream(v.device), res),
           "arnoldi_exp")

@@ -626,10 +626,12 @@ def _pipelined_arnoldi(op, v: torch.Tensor, exp_tolerance: float, norm_tolerance
 
 
 # Up to this many qubits the Arnoldi iteration runs in the library's driver
-# (emu_sv_arnoldi_exp; its basis buffer holds max_krylov_dim vectors): there
-# the per-iteration work is a few tens of microseconds of GPU time, which
-# interpreter work per iteration would otherwise exceed.
-NATIVE_ARNOLDI_MAX_QUBITS = 16
+# (emu_sv_arnoldi_exp, with a basis of max_krylov_dim vectors from torch's
+# allocator for the call: 1.6 GiB at 20 complex128 qubits): there the
+# per-iteration GPU work is short enough that the interpreter's work per
+# iteration would show (a 50-step N=18 trajectory 0.32 -> 0.24 s; at N=20
+# the two engines tie).
+NATIVE_ARNOLDI_MAX_QUBITS = 20
 
 
 def _native_arnoldi(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,

@@ -215,7 +215,7 @@ def test_arnoldi_engines_agree(gpu_device, monkeypatch, n, dt, tol, dtype):
         assert diff <= bound * scale, diff
     diff = float((native.result - piped.result).abs().max())
     assert diff <= (1e-12 if dtype == torch.complex128 else 1e-6) * scale, diff
-    # the dispatcher takes the native driver for n <= 16
+    # the dispatcher takes the native driver (n <= 20)
     monkeypatch.setattr(kx, "PIPELINE", True)
     calls = []
     real = kx._native_arnoldi
