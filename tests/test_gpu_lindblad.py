@@ -114,7 +114,7 @@ def test_pair_kernel_vs_dense(gpu_device, n, a, b):
     assert torch.allclose(out.cpu(), full @ v, atol=1e-12, rtol=0)
 
 
-@pytest.mark.parametrize("n,dt,tol", [(2, 0.5, 1e-10), (3, 0.3, 1e-12), (5, 0.2, 1e-10)])
+@pytest.mark.parametrize("n,dt,tol", [(2, 0.5, 1e-10), (3, 0.3, 1e-12), (5, 0.2, 1e-10), (7, 0.1, 1e-10)])
 def test_density_step_vs_oracle(gpu_device, n, dt, tol):
     from emulators_amd.emu_base.math.krylov_exp import krylov_exp_impl
     from emulators_amd.emu_sv.lindblad_operator import RydbergLindbladian, ScaledLindbladian
