@@ -230,18 +230,29 @@ __device__ __forceinline__ typename Vec2<T>::type load_input(const PassArgs &a, 
 // NOISE (with CPLX): the block's single-qubit term K is added, K10 / K01 to
 // the flip coefficients and (n - popcount(x)) K00 + popcount(x) K11 to the
 // diagonal of amplitude x.
+// The static LDS of one pass_tile, declared once by the calling kernel: a
+// kernel that inlines several pass_tile variants (the batch engine's) then
+// holds one copy instead of one per variant.
+struct PassLds {
+  double ap[kMaxTileLog];
+  double U[kMaxTileLog][kMaxTileLog];
+  double B[1 << (kMaxTileLog - kThreadBits)];
+  double eout;
+  double red[kWaves * kNumDots];
+};
+
 template <typename T, bool CPLX, bool DOTS, bool NOISE = false>
 __device__ __forceinline__ void pass_tile(const PassArgs &a, uint64_t g, const double *row,
-                                          uint64_t prow) {
+                                          uint64_t prow, PassLds &L) {
   static_assert(!NOISE || CPLX, "the noise term needs complex coefficients");
   using T2 = typename Vec2<T>::type;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T2 *tile = reinterpret_cast<T2 *>(smem_raw);
-  __shared__ double s_ap[kMaxTileLog];
-  __shared__ double s_U[kMaxTileLog][kMaxTileLog];
-  __shared__ double s_B[1 << (kMaxTileLog - kThreadBits)];
-  __shared__ double s_eout;
-  __shared__ double s_red[kWaves * kNumDots];
+  double *s_ap = L.ap;
+  auto &s_U = L.U;
+  double *s_B = L.B;
+  double &s_eout = L.eout;
+  double *s_red = L.red;
 
   const int tid = threadIdx.x;
   const int c = a.c, s = a.s, m = a.m, n = a.n;
@@ -497,7 +508,8 @@ __global__ __launch_bounds__(kThreads) void hpsi_pass_kernel(const PassArgs args
   const uint64_t ntiles = 1ull << (a.n - t);
   uint64_t g = blockIdx.x;
   if (a.xcd && ntiles >= 8) g = (g & 7) * (ntiles >> 3) + (g >> 3);
-  pass_tile<T, CPLX, DOTS, NOISE>(a, g, a.diag ? a.dtab + g * kDiagRow : nullptr, blockIdx.x);
+  __shared__ PassLds lds;
+  pass_tile<T, CPLX, DOTS, NOISE>(a, g, a.diag ? a.dtab + g * kDiagRow : nullptr, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -4150,7 +4162,8 @@ int emu_sv_batch_create(int count, int dtype, const emu_sv_batch_register *regs,
   }
   h->ncomb = L.ncomb;
   h->comb = tables + coff;
-  h->lds = esz << max_t;
+  // the first pass stages U in the same dynamic area before its tile
+  h->lds = std::max(esz << max_t, kBatchUStageBytes);
   int rc = EMU_SV_OK;
   if (h->lds > 65536) {  // dynamic LDS above 64 KiB is enabled per kernel
     rc = hip_check(hipFuncSetAttribute(dtype == EMU_SV_C128
