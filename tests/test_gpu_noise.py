@@ -467,3 +467,53 @@ def test_monte_carlo_step_n16_vs_oracle(gpu_device, monkeypatch, schedule):
         diff = float((psi.cpu() - ref["states"][s]).abs().max())
         assert diff < 1e-8, f"step {s}: {diff}"
     assert mc.jump_threshold == pytest.approx(ref["threshold"], rel=1e-9)
+
+
+def test_concurrent_streams_arnoldi_drivers(gpu_device):
+    """Two host threads, each on its own torch stream, run noisy Krylov steps
+    through the library's Arnoldi driver at the same time: one register in
+    the Gram-matrix form (n = 14), one in the one-workgroup Gram-Schmidt
+    (n = 12).  Every buffer of a run is per stream or per call, so both
+    results equal the oracle's."""
+    import threading
+
+    from emulators_amd.emu_base.math.krylov_exp import ScaledHamiltonian, krylov_exp
+    from emulators_amd.emu_sv.hamiltonian import RydbergHamiltonian
+    k = orc.noise_from_lindbladians(LINDBLADS)
+    dt, tol, steps = 0.05, 1e-10, 4
+    cases = []
+    for n in (14, 12):
+        om, de, ph, u = _params(n, 90 + n, True)
+        psi = torch.randn(2**n, dtype=torch.complex128, generator=torch.Generator().manual_seed(n))
+        cases.append((n, om, de, ph, u, psi / psi.norm()))
+    results: dict = {}
+    errors: list = []
+    barrier = threading.Barrier(2)
+
+    def worker(i):
+        try:
+            n, om, de, ph, u, psi = cases[i]
+            stream = torch.cuda.Stream(device=gpu_device)
+            with torch.cuda.stream(stream):
+                ham = RydbergHamiltonian(om, de, ph, u, device=gpu_device, noise=k)
+                v = psi.to(gpu_device)
+                barrier.wait()
+                for _ in range(steps):
+                    v = krylov_exp(ScaledHamiltonian(ham, -1j * dt), v, tol, tol, is_hermitian=False)
+                stream.synchronize()
+                results[i] = v.cpu()
+        except Exception as exc:  # reported by the main thread
+            errors.append((i, repr(exc)))
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not errors, errors
+    for i, (n, om, de, ph, u, psi) in enumerate(cases):
+        ho = orc.OracleHamiltonian(om, de, ph, u, noise=k)
+        ref = psi.clone()
+        for _ in range(steps):
+            ref = orc.krylov_exp(lambda x: -1j * dt * (ho * x), ref.clone(), tol, tol, is_hermitian=False)
+        assert float((results[i] - ref).abs().max()) < 1e-9, (n, float((results[i] - ref).abs().max()))
