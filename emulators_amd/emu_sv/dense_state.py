@@ -108,9 +108,35 @@ class DenseState:
     @classmethod
     def from_state_amplitudes(cls: Type[DenseStateType], *, eigenstates: Sequence[str],
                               amplitudes: Mapping[str, complex]) -> DenseStateType:
-        """State from a {bitstring: amplitude} mapping, normalised."""
-        n = len(next(iter(amplitudes)))
+        """State from a {bitstring: amplitude} mapping, normalised.  The
+        bitstrings must all have one length and use only the eigenstates'
+        labels (the check pulser's State.from_state_amplitudes makes before
+        dense_state.py:192-240 runs): ValueError otherwise."""
+        if not amplitudes:
+            raise ValueError("at least one amplitude is needed")
+        lengths = {len(k) for k in amplitudes}
+        if len(lengths) != 1:
+            raise ValueError(f"every bitstring must have the same length, got {sorted(lengths)}")
+        labels = set(eigenstates)
+        bad = sorted({c for k in amplitudes for c in k} - labels)
+        if bad:
+            raise ValueError(f"bitstrings use labels {bad} outside the eigenstates {tuple(eigenstates)}")
+        n = lengths.pop()
         state, amps = cls._from_state_amplitudes(eigenstates=eigenstates, n_qudits=n,
                                                  amplitudes=amplitudes)
-        state._amplitudes = dict(amps)  # kept for the config's serialised form
+        state._amplitudes = dict(amps)  # kept for the serialised form
         return state
+
+    def _to_abstract_repr(self) -> dict:
+        """{"eigenstates": ..., "amplitudes": {bitstring: amplitude}}: the
+        amplitudes the state was built from, else its nonzero amplitudes
+        (state vectors only)."""
+        amps = getattr(self, "_amplitudes", None)
+        if amps is None:
+            data = self.data.detach().cpu()
+            if data.dim() != 1:
+                raise ValueError("only state vectors can be serialised")
+            n = self.n_qudits  # bit value 1 = "r", 0 = "g", as _build_state_vector reads them
+            amps = {format(i, f"0{n}b").replace("1", "r").replace("0", "g"): complex(v)
+                    for i, v in enumerate(data.tolist()) if v != 0}
+        return {"eigenstates": tuple(self.eigenstates), "amplitudes": dict(amps)}

@@ -137,16 +137,9 @@ class SVConfig:
 
 
 def _state_repr(state) -> dict:
-    amps = getattr(state, "_amplitudes", None)
-    if amps is None:  # from the nonzero amplitudes of the tensor
-        data = state.data.detach().cpu()
-        if data.dim() != 1:
-            raise ValueError("only state vectors can be serialised")
-        n = int(data.numel()).bit_length() - 1
-        amps = {format(i, f"0{n}b").replace("1", "r").replace("0", "g"): complex(v)
-                for i, v in enumerate(data.tolist()) if v != 0}
-    return {"eigenstates": list(state.eigenstates),
-            "amplitudes": {k: _amplitude_repr(v) for k, v in amps.items()}}
+    d = state._to_abstract_repr()
+    return {"eigenstates": list(d["eigenstates"]),
+            "amplitudes": {k: _amplitude_repr(v) for k, v in d["amplitudes"].items()}}
 
 
 def _amplitude_repr(v):
