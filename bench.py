@@ -721,6 +721,7 @@ def main() -> int:
                        "passes": passes64, "schedule": sched64.kwargs()}
     log(f"Lanczos matvec c128 {ms128:.3f} ms ({gbs128:.0f} GB/s algorithmic)")
     copy_gbs, copy_ms = copy_bandwidth(n, args.hpsi_reps)
+    traffic128 = measured_traffic("c128", sched128, n)
     log(f"copy kernel: {copy_gbs:.0f} GB/s ({copy_ms:.3f} ms per 2 x {16 * 2**n} bytes)")
 
     adjoint = None if args.no_adjoint else adjoint_cost()
@@ -784,7 +785,11 @@ def main() -> int:
             "frac_of_attainable": gbs128 / copy_gbs,
             "attainable_kernel": "plain device copy of one complex128 vector (emu_sv_convert), "
                                  "HIP events, same run",
-            "traffic": measured_traffic("c128", sched128, n),
+            "traffic": traffic128,
+            # the HBM rate the iteration actually sustains: counter bytes per call / call time
+            "traffic_GBps": None if traffic128 is None else traffic128 / (ms128 * 1e-3) / 1e9,
+            "traffic_frac_of_attainable": (None if traffic128 is None
+                                           else traffic128 / (ms128 * 1e-3) / 1e9 / copy_gbs),
             "kernel": f"one Lanczos iteration (emu_sv_lanczos_matvec, complex128): "
                       f"orthogonalise + {passes128} H.psi passes ({sched128.kernel} kernel, "
                       f"{tile_text(sched128, 14)}) + reductions; algorithmic bytes 3*D*16 per call "
