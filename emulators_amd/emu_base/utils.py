@@ -71,16 +71,26 @@ def get_max_rss_gpu() -> float:
     return max(peaks) * 1e-6
 
 
+def readout_with_error(c: str, *, p_false_pos: float, p_false_neg: float) -> str:
+    """One bit read out with errors (reference emu_base/utils.py:102-111):
+    one uniform draw per bit, '0' reads as '1' below ``p_false_pos``, '1'
+    reads as '0' below ``p_false_neg``."""
+    r = random.random()
+    if c == "0":
+        return "1" if r < p_false_pos else c
+    if c == "1":
+        return "0" if r < p_false_neg else c
+    return c
+
+
 def apply_measurement_errors(bitstrings: Counter, *, p_false_pos: float,
                              p_false_neg: float) -> Counter:
-    """Resample a bag of bitstrings with readout errors: a '0' reads as '1'
-    with probability ``p_false_pos``, a '1' as '0' with ``p_false_neg``."""
-    threshold = {"0": p_false_pos, "1": p_false_neg}
-    flipped = {"0": "1", "1": "0"}
+    """Resample a bag of bitstrings with readout errors, every bit of every
+    shot through ``readout_with_error`` in order (reference
+    emu_base/utils.py:114-136), so a seeded ``random`` gives the same bag."""
     out: Counter = Counter()
     for bits, count in bitstrings.items():
         for _ in range(count):
-            shot = [flipped[ch] if random.random() < threshold.get(ch, 0.0) else ch
-                    for ch in bits]
-            out["".join(shot)] += 1
+            out["".join(readout_with_error(ch, p_false_pos=p_false_pos,
+                                           p_false_neg=p_false_neg) for ch in bits)] += 1
     return out

@@ -247,6 +247,35 @@ def test_lindblad_operators_eff_noise_basis():
             [[0.0, 1.0], [0.0, 0.0]], dtype=torch.complex128))
 
 
+def test_lindblad_operators_eff_noise_flip_3x3():
+    """test_lindblad_operators.py:43-77: for 3x3 operators in the ising basis
+    only the leading 2x2 block is flipped (both axes), the rest is scaled by
+    sqrt(rate) and kept; a 3x3 numpy operator at dim 2 raises
+    (test_lindblad_operators.py:80-91)."""
+    import numpy as np
+    from emulators_amd.emu_base.jump_lindblad_operators import get_lindblad_operators
+    g = torch.Generator().manual_seed(42)
+    a0 = torch.rand(3, 3, dtype=torch.complex128, generator=g)
+    a1 = torch.rand(3, 3, dtype=torch.complex128, generator=g)
+    nm = _noise_model(noise_types=("eff_noise",), eff_noise_rates=[0.5, 0.5],
+                      eff_noise_opers=[a0, a1])
+    ops = get_lindblad_operators(noise_type="eff_noise", noise_model=nm, interact_type="ising",
+                                 dim=3)
+    assert len(ops) == 2
+    for op, a in zip(ops, (a0, a1)):
+        s = math.sqrt(0.5)
+        # element by element: (i, j) of the 2x2 block comes from (1-i, 1-j)
+        for i in range(3):
+            for j in range(3):
+                src = (1 - i, 1 - j) if i < 2 and j < 2 else (i, j)
+                assert torch.allclose(op[i, j], s * a[src])
+    b0 = np.array([0.0, 1.0, 0.0]).reshape(3, 1)
+    b1 = np.array([1.0, 0.0, 0.0]).reshape(3, 1)
+    with pytest.raises(ValueError, match="Only 2 by 2 effective noise operator matrices"):
+        get_lindblad_operators(noise_type="eff_noise", noise_model=_noise_model(
+            noise_types=("eff_noise",), eff_noise_rates=[0.5], eff_noise_opers=[b0 @ b1.T]))
+
+
 def test_lindblad_operators_channels():
     """Relaxation / dephasing / depolarizing closed forms
     (jump_lindblad_operators.py:18-54); non-Lindbladian noise types are
