@@ -2,6 +2,7 @@
 
 from .dense_state import DenseState
 from .density_matrix_state import DensityMatrix
+from .matrix_operator import DenseOperator, SparseOperator
 from .hamiltonian import RydbergHamiltonian
 from .lindblad_operator import RydbergLindbladian
 from .observables import (
@@ -10,6 +11,7 @@ from .observables import (
     Energy,
     EnergySecondMoment,
     EnergyVariance,
+    Expectation,
     Fidelity,
     Observable,
     Occupation,
@@ -23,11 +25,13 @@ from .sv_config import Solver, SVConfig
 __all__ = [
     "BitStrings",
     "CorrelationMatrix",
+    "DenseOperator",
     "DenseState",
     "DensityMatrix",
     "Energy",
     "EnergySecondMoment",
     "EnergyVariance",
+    "Expectation",
     "Fidelity",
     "Observable",
     "Occupation",
@@ -36,6 +40,7 @@ __all__ = [
     "RydbergLindbladian",
     "SVBackend",
     "SVConfig",
+    "SparseOperator",
     "Solver",
     "StateResult",
     "StateVector",

@@ -125,6 +125,20 @@ class Fidelity(Observable):
         return self.state.overlap(state)
 
 
+class Expectation(Observable):
+    """<psi|O|psi> of an operator (pulser's ``Expectation``): the operator's
+    own ``expect``, a complex128 scalar tensor."""
+
+    _base_tag = "expectation"
+
+    def __init__(self, operator, **kwargs):
+        super().__init__(**kwargs)
+        self.operator = operator
+
+    def apply(self, *, config, state, hamiltonian):
+        return self.operator.expect(state)
+
+
 class StateResult(Observable):
     _base_tag = "state"
 

@@ -14,7 +14,8 @@ Lanczos coefficients, reductions and the small exponential stay FP64).
 
 ``to_abstract_repr`` / ``from_abstract_repr`` serialise a config as JSON
 (sv_config.py and pulser's EmulationConfig): the keyword arguments, the
-observables (type, tag suffix, evaluation times and their own options) and
+observables (type, tag suffix, evaluation times and their own options; an
+``Expectation``'s operator as its class and operator representation) and
 an initial state as eigenstates plus {bitstring: amplitude}.  A real
 amplitude is written as a plain number, so the parsed JSON compares equal
 to the amplitudes given (test/emu_sv/test_svconfig.py:76-98); a complex
@@ -169,7 +170,32 @@ def _observable_repr(obs) -> dict:
         d["num_shots"] = obs.num_shots
     if type(obs).__name__ == "Fidelity":
         d["state"] = _state_repr(obs.state)
+    if type(obs).__name__ == "Expectation":
+        d["operator"] = _operator_repr(obs.operator)
     return d
+
+
+def _operator_repr(op) -> dict:
+    """An operator built by from_operator_repr: its class and representation
+    (complex weights as plain numbers or real/imag objects)."""
+    r = op._to_abstract_repr()
+    terms = [[_amplitude_repr(coeff),
+              [[{k: _amplitude_repr(w) for k, w in qop.items()}, sorted(int(t) for t in targets)]
+               for qop, targets in factors]]
+             for coeff, factors in r["operations"]]
+    return {"type": type(op).__name__, "eigenstates": r["eigenstates"],
+            "n_qudits": r["n_qudits"], "operations": terms}
+
+
+def _operator_from_repr(d: dict):
+    from emulators_amd.emu_sv import matrix_operator
+    cls = getattr(matrix_operator, d["type"])
+    ops = [(_amplitude_from_repr(coeff),
+            [({k: _amplitude_from_repr(w) for k, w in qop.items()}, list(targets))
+             for qop, targets in factors])
+           for coeff, factors in d["operations"]]
+    return cls.from_operator_repr(eigenstates=tuple(d["eigenstates"]),
+                                  n_qudits=int(d["n_qudits"]), operations=ops)
 
 
 def _observable_from_repr(d: dict):
@@ -180,4 +206,6 @@ def _observable_from_repr(d: dict):
         kw["num_shots"] = d["num_shots"]
     if d["type"] == "Fidelity":
         return cls(_state_from_repr(d["state"]), **kw)
+    if d["type"] == "Expectation":
+        return cls(_operator_from_repr(d["operator"]), **kw)
     return cls(**kw)
