@@ -36,7 +36,7 @@ def expect_batch(state_vector: torch.Tensor, single_qubit_operators: torch.Tenso
 
 def apply(state_vector: torch.Tensor, qubit_index: int,
           single_qubit_operator: torch.Tensor) -> torch.Tensor:
-    """The operator on one qubit, in place on the device state
-    (algebra.py:28-37)."""
+    """The operator on one qubit, as a new device vector; the input is left
+    as it is (algebra.py:28-37)."""
     n = native.nqubits_of(state_vector)
-    return native.apply_1q(n, state_vector, qubit_index, single_qubit_operator)
+    return native.apply_1q(n, state_vector.clone(), qubit_index, single_qubit_operator)

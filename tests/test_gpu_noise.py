@@ -302,8 +302,9 @@ def test_jump_helpers_vs_dense(gpu_device, dtype):
     tol = 1e-11 if dtype == torch.complex128 else 2e-5 * float(psi.norm() ** 2)
     assert float((got - ref).abs().max()) < tol
     for q in (0, 3, 5):
-        d = psi.to(gpu_device, dtype).clone()
-        apply(d, q, LINDBLADS[2])
+        src = psi.to(gpu_device, dtype)
+        d = apply(src, q, LINDBLADS[2])
+        assert torch.equal(src.cpu(), psi.to(dtype))  # the input is left as it was
         exp_ = orc.apply_1q(psi, q, LINDBLADS[2])
         assert float((d.cpu().to(torch.complex128) - exp_).abs().max()) < (
             1e-13 if dtype == torch.complex128 else 1e-5)

@@ -324,3 +324,25 @@ def test_inverse_cdf_indices_brute_force():
     point[4] = 2.5
     assert torch.equal(orc.inverse_cdf_indices(point, torch.linspace(0, 0.999, 50, dtype=torch.float64)),
                        torch.full((50,), 4, dtype=torch.int64))
+
+
+# the reference's known answers for the jump helpers (test/emu_sv/test_algebra.py:5-29)
+ALGEBRA_VEC = [0.0, 0.0, 0.5j, 1.0]
+ALGEBRA_OPS = [[[0.0, 0.0], [0.0, 1.0j]], [[1.0, 0.0], [0.0, 0.0]], [[0.0, 1.0j], [0.0, 0.0]]]
+ALGEBRA_EXPECT = [[1.25j, 0.0, 0.0], [1.0j, 0.25, 0.5]]
+HADAMARD_LIKE = [[1.0, -1.0], [1.0, 1.0]]
+APPLY_VEC = [0.0, 0.0, 0.5, 1.0]
+APPLY_Q0 = [-0.5, -1.0, 0.5, 1.0]
+APPLY_Q1 = [0.0, 0.0, -0.5, 1.5]
+
+
+def test_oracle_jump_helpers_known_answers():
+    """test_algebra.py:5-29: Tr(A_i rho_q) on an unnormalised 2-qubit vector,
+    and [[1, -1], [1, 1]] on qubit 0 and on qubit 1."""
+    c = torch.complex128
+    got = orc.expect_batch(torch.tensor(ALGEBRA_VEC, dtype=c), torch.tensor(ALGEBRA_OPS, dtype=c), 2)
+    assert torch.allclose(got, torch.tensor(ALGEBRA_EXPECT, dtype=c))
+    vec = torch.tensor(APPLY_VEC, dtype=c)
+    had = torch.tensor(HADAMARD_LIKE, dtype=c)
+    assert torch.allclose(orc.apply_1q(vec, 0, had), torch.tensor(APPLY_Q0, dtype=c))
+    assert torch.allclose(orc.apply_1q(vec, 1, had), torch.tensor(APPLY_Q1, dtype=c))
