@@ -1820,7 +1820,8 @@ __global__ __launch_bounds__(kThreads) void copy_packets_kernel(uint64_t npk, co
   }
 }
 
-constexpr int kLincombBatch = 48;  // vectors per launch (kernel-argument size)
+constexpr int kLincombBatch = 112;  // vectors per launch: 2.7 KiB of kernel arguments, so a
+                                    // whole Krylov basis (<= 101 vectors) is one launch
 constexpr int kMaxLincomb = 1024;
 struct LincombArgs {
   const void *vec[kLincombBatch];
