@@ -31,7 +31,9 @@ tests/test_noise_host.py, tests/test_lindblad_host.py):
     evaluation counts 12, 13, 18) and a dense-expm jump-time root.
   * expect_batch / apply_1q: dense Kronecker-product operators.
   * Monte-Carlo step: a single decaying atom, whose jump time has the closed
-    form exp(-gamma t) = threshold.
+    form exp(-gamma t) = threshold; and the reference's seeded known answer
+    of ten two-atom emission trajectories (mean occupations [0.5, 0.2],
+    test_end_to_end.py:458-545), which fixes the draw order.
   * Lindbladian: the textbook generator -i[H, rho] + sum (A rho A^H -
     1/2 {A^H A, rho}) as a dense 4^N superoperator, the reference's no-jump
     and Tr(H rho) checks, and the e^{-gamma t} decay of one atom.

@@ -202,6 +202,62 @@ def test_oracle_monte_carlo_decay_closed_form():
     assert abs(abs(psi[0]) - 1) < 1e-12 and abs(psi[1]) < 1e-12
 
 
+@pytest.mark.usefixtures("restore_rng_state")
+def test_oracle_monte_carlo_emission_known_answer():
+    """The reference's seeded Monte-Carlo answer of
+    test_end_to_end_spontaneous_emission_rate (test/emu_sv/test_end_to_end.py:
+    458-545): two non-interacting atoms, no drive, relaxation 0.1 / us for
+    10 us (100 steps of 100 ns) from |rr>, Krylov tolerance 1e-5, 10
+    trajectories after random.seed(0xDEADBEEF): mean final occupations
+    [0.5, 0.2].  Each trajectory ends with each atom in |g> or |r>, so the
+    answer is a count of which atoms jumped; it fixes the threshold draws,
+    the random.choices of the jumping qubit and the jump-time search to the
+    reference's draw order."""
+    random.seed(0xDEADBEEF)
+    relax = [math.sqrt(0.1) * torch.tensor([[0, 1], [0, 0]], dtype=torch.complex128)]
+    zero = torch.zeros(2, dtype=torch.float64)
+    occs = []
+    for _ in range(10):
+        mc = orc.OracleMonteCarlo()
+        psi = torch.zeros(4, dtype=torch.complex128)
+        psi[3] = 1.0  # |rr>
+        for _ in range(100):
+            psi = mc.step(0.1, zero, zero, zero, torch.zeros(2, 2, dtype=torch.float64), psi, 1e-5, relax)
+        occs.append(orc.occupation(psi / psi.norm()))
+    mean = torch.stack(occs).mean(0)
+    assert torch.allclose(mean, torch.tensor([0.5, 0.2], dtype=torch.float64), atol=1e-12)
+
+
+@pytest.mark.usefixtures("restore_rng_state")
+def test_oracle_monte_carlo_random_effective_noise_known_answer():
+    """The reference's test_end_to_end_random_effective_noise
+    (test/emu_sv/test_end_to_end.py:548-600): torch.manual_seed(31415) draws
+    the effective-noise operator (torch.rand(2, 2), rate 30, pulser basis
+    flipped to the state-vector one), two atoms 8 um apart (U = C6 / 8^6), no
+    drive, 1000 ns in 100 steps from |rr>, Krylov tolerance 1e-10, 10
+    trajectories after random.seed(0xDEADBEEF): mean final occupations
+    [0.8508, 0.8536] (its atol 1e-4)."""
+    from emulators_amd.emu_base.jump_lindblad_operators import get_all_lindblad_noise_operators
+    torch.manual_seed(31415)
+    random.seed(0xDEADBEEF)
+    op = torch.rand(2, 2, dtype=torch.float64, device="cpu")
+    ops = get_all_lindblad_noise_operators(_noise_model(noise_types=("eff_noise",),
+                                                        eff_noise_rates=(30.0,), eff_noise_opers=(op,)))
+    u = 5420158.53 / 8.0**6
+    umat = torch.tensor([[0.0, u], [u, 0.0]], dtype=torch.float64)
+    zero = torch.zeros(2, dtype=torch.float64)
+    occs = []
+    for _ in range(10):
+        mc = orc.OracleMonteCarlo()
+        psi = torch.zeros(4, dtype=torch.complex128)
+        psi[3] = 1.0  # |rr>
+        for _ in range(100):
+            psi = mc.step(0.01, zero, zero, zero, umat, psi, 1e-10, ops)
+        occs.append(orc.occupation(psi / psi.norm()))
+    mean = torch.stack(occs).mean(0)
+    assert torch.allclose(mean, torch.tensor([0.8508, 0.8536], dtype=torch.float64), atol=1e-4)
+
+
 @pytest.mark.parametrize("solver,ops,stochastic,expected", [
     ("default", [], False, "EvolveStateVector"),
     ("default", [1], True, "EvolveMonteCarlo"),
