@@ -184,10 +184,20 @@ class Results:
         self._aggregation[tag] = observable.default_aggregation_method
         self._results.setdefault(tag, {})[float(time)] = value
 
-    def get_result_times(self, tag: str) -> list[float]:
-        return sorted(self._results[tag].keys())
+    def _tag_of(self, observable) -> str:
+        # an observable or its tag, as pulser's Results accepts either
+        if isinstance(observable, str):
+            return observable
+        tag = observable.tag
+        if self._tagmap.get(tag) not in (None, observable.uuid):
+            raise ValueError(f"the results of {tag!r} belong to another observable")
+        return tag
 
-    def get_result(self, tag: str, time: float) -> Any:
+    def get_result_times(self, observable) -> list[float]:
+        return sorted(self._results[self._tag_of(observable)].keys())
+
+    def get_result(self, observable, time: float) -> Any:
+        tag = self._tag_of(observable)
         times = self._results[tag]
         best = min(times, key=lambda t: abs(t - time))
         if abs(best - time) > 1e-10:

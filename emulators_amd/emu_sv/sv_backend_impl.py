@@ -139,6 +139,11 @@ class SVBackendImpl:
                 return mat
 
             self.interaction_matrix = interaction_matrix
+            # zeroed on copies: the sequence data's tables may be shared with
+            # other runs (the same SequenceData with other bad atoms)
+            self.omega = self.omega.clone()
+            self.delta = self.delta.clone()
+            self.phi = self.phi.clone()
             self.omega[:, bad] = 0.0
             self.delta[:, bad] = 0.0
             self.phi[:, bad] = 0.0

@@ -132,3 +132,106 @@ def test_random_effective_noise_trajectories(gpu_device):
     runs = [SVBackend._run_from_sequence_data(seq, cfg) for _ in range(10)]
     occ = torch.stack([torch.as_tensor(r.occupation[-1]).cpu() for r in runs]).mean(0)
     assert torch.allclose(occ, torch.tensor([0.8508, 0.8536], dtype=torch.float64), atol=1e-4)
+
+
+def test_trajectories_with_badly_prepared_atoms(gpu_device):
+    """The reference's test_trajectories (test/emu_sv/test_end_to_end.py:
+    973-1003): three atoms 1e5 um apart, a pi pulse, state-preparation error
+    1/3 with one badly prepared atom per trajectory (the reference mocks the
+    draws to make atoms 0, 1, 2 bad in turn).  A bad atom's drive and
+    interactions are switched off (sv_backend_impl.py:162-183), so it stays in
+    |g> while the others end in |r>: the aggregated occupations are 2/3 each
+    (its 0.6667 at atol 1e-4)."""
+    import dataclasses
+
+    from emulators_amd.emu_sv import Occupation, Results
+    steps, step_ns = 10, 10.0
+    omega = math.pi / (steps * step_ns * 1e-3)  # area pi
+    seq = workloads.piecewise_sequence(np.array([[-1e5, 0.0], [1e5, 0.0], [0.0, 1e5]]),
+                                       np.full(steps, omega), np.zeros(steps), step_ns=step_ns)
+    cfg = SVConfig(observables=[Occupation(evaluation_times=[1.0])], log_level=40)
+    omega_in = seq.omega.clone()
+    runs = []
+    for bad in range(3):  # the three runs share seq's tables
+        flags = tuple(q == bad for q in range(3))
+        sd = dataclasses.replace(seq, bad_atoms=flags, state_prep_error=1.0 / 3.0)
+        r = SVBackend._run_from_sequence_data(sd, cfg)
+        occ = torch.as_tensor(r.occupation[-1]).cpu()
+        want = torch.tensor([0.0 if f else 1.0 for f in flags], dtype=torch.float64)
+        assert torch.allclose(occ, want, atol=1e-9), (bad, occ)
+        runs.append(r)
+    assert torch.equal(seq.omega, omega_in)  # the caller's tables are left alone
+    agg = torch.as_tensor(Results.aggregate(runs).occupation[-1]).cpu()
+    assert torch.allclose(agg, torch.full((3,), 2.0 / 3.0, dtype=torch.float64), atol=1e-9)
+    assert torch.allclose(agg, torch.full((3,), 0.6667, dtype=torch.float64), atol=1e-4)
+
+
+def test_observable_times_as_in_pulser(gpu_device):
+    """The reference's test_end_to_end_observable_time_as_in_pulser
+    (test/emu_sv/test_end_to_end.py:949-970): a 20 ns constant pulse
+    (Omega = 5) on two atoms 6 um apart, dt = 2.5 ns, occupation requested
+    at [0, 1/3, 1]: the run reports exactly those times, 1/3 (6.67 ns) being
+    off the dt grid (a shorter step lands on it, pulser_adapter.py:130-152)."""
+    from emulators_amd.emu_base.pulser_adapter import (interaction_matrix_from_positions,
+                                                        sequence_data_from_samples)
+    from emulators_amd.emu_sv import Occupation
+    duration, dt = 20, 2.5
+    occ = Occupation(evaluation_times=[0.0, 1 / 3, 1.0])
+    cfg = SVConfig(dt=dt, observables=[occ], log_level=40)
+    samples = {q: {"amp": [5.0] * duration, "det": [0.0] * duration, "phase": [0.0] * duration}
+               for q in ("q0", "q1")}
+    pos = np.array([[-3.0, 0.0], [3.0, 0.0]])
+    seq = sequence_data_from_samples(samples, ("q0", "q1"),
+                                     interaction_matrix_from_positions(pos, workloads.C6), cfg, dt)
+    assert any(abs(t - duration / 3) < 1e-9 for t in seq.target_times)
+    res = SVBackend._run_from_sequence_data(seq, cfg)
+    assert np.allclose(res.get_result_times(occ), [0.0, 1 / 3, 1.0])
+    # the two atoms are equivalent: equal occupations at every reported time
+    for o in res.occupation:
+        o = torch.as_tensor(o).cpu()
+        assert abs(float(o[0] - o[1])) < 1e-10
+
+
+def test_initial_state_is_used_and_copied(gpu_device):
+    """The reference's test_initial_state (test/emu_sv/test_end_to_end.py:
+    336-358): a pulse that does nothing on two atoms 1e10 um apart, initial
+    state |rr>: the final StateResult is that state (overlap 1) and a copy."""
+    from emulators_amd.emu_sv import StateVector
+    steps = 1
+    seq = workloads.piecewise_sequence(np.array([[0.0, 0.0], [1e10, 0.0]]), np.zeros(steps),
+                                       np.zeros(steps), step_ns=10.0)
+    state = StateVector.from_state_amplitudes(eigenstates=("r", "g"), amplitudes={"rr": 1.0})
+    assert abs(float(state.norm()) - 1.0) < 1e-12
+    sr = StateResult(evaluation_times=[1.0])
+    res = SVBackend._run_from_sequence_data(seq, SVConfig(observables=[sr], initial_state=state,
+                                                          log_level=40))
+    out = res.get_result(sr, 1.0)
+    assert abs(complex(out.inner(state)).real - 1.0) < 1e-12
+    assert out is not state
+
+
+def test_initial_density_matrix_under_dephasing(gpu_device):
+    """The reference's test_initial_state_with_den_matrix (test/emu_sv/
+    test_end_to_end.py:385-419): five atoms 1e10 um apart, no drive,
+    dephasing 0.5 / us, initial density matrix |rrrrr><rrrrr| on the CPU: the
+    final StateResult equals it (atol 1e-8; dephasing leaves a basis state
+    alone) and is a copy."""
+    import dataclasses
+
+    from emulators_amd.emu_sv import DensityMatrix
+    from emulators_amd.emu_sv.sv_config import Solver
+    n, steps = 5, 1
+    seq = workloads.piecewise_sequence(np.array([[1e10 * q, 0.0] for q in range(n)]), np.zeros(steps),
+                                       np.zeros(steps), step_ns=10.0)
+    c = math.sqrt(0.5 / 2)  # dephasing sqrt(rate / 2) Z
+    seq = dataclasses.replace(seq, lindblad_ops=[torch.tensor([[c, 0], [0, -c]], dtype=torch.complex128)],
+                              has_stochastic_noise=False)
+    state = DensityMatrix.from_state_amplitudes(eigenstates=("r", "g"), amplitudes={"r" * n: 1.0})
+    state.data = state.data.to("cpu")
+    assert abs(complex(state.data.trace()) - 1.0) < 1e-12
+    sr = StateResult(evaluation_times=[1.0])
+    res = SVBackend._run_from_sequence_data(seq, SVConfig(observables=[sr], initial_state=state,
+                                                          solver=Solver.LINDBLAD, log_level=40))
+    out = res.get_result(sr, 1.0)
+    assert torch.allclose(out.data.cpu(), state.data.cpu(), atol=1e-8)
+    assert out is not state

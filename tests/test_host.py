@@ -287,8 +287,15 @@ def test_results_protocol():
     assert res.get_result_times("energy") == [0.0, 0.5, 1.0]
     assert [float(x) for x in res.energy] == [1.0, 3.0, 4.0]
     assert float(res.get_result("energy", 0.5)) == 3.0
+    # the observable itself selects its results too, as in pulser's Results
+    # (the reference's tests call results.get_result(state_result, 1.0))
+    assert res.get_result_times(e) == [0.0, 0.5, 1.0]
+    assert float(res.get_result(e, 1.0)) == 4.0
+    with pytest.raises(ValueError, match="another observable"):
+        res.get_result(Energy(evaluation_times=[1.0]), 1.0)
     agg = Results.aggregate([res, res])
     assert [float(x) for x in agg.energy] == [1.0, 3.0, 4.0]
+    assert float(agg.get_result(e, 0.0)) == 1.0
 
 
 def test_sampling_helpers():
