@@ -1,7 +1,10 @@
 """Shared pieces of the state-vector path (mirrors the reference's emu_base)."""
 
 from .constants import DEVICE_COUNT
+from .jump_lindblad_operators import compute_noise_from_lindbladians
+from .math.brents_root_finding import find_root_brents
 from .math.krylov_exp import DEFAULT_MAX_KRYLOV_DIM, krylov_exp
+from .math.matmul import matmul_2x2_with_batched
 from .pulser_adapter import (
     HamiltonianType,
     InteractionMatrixCallable,
@@ -30,6 +33,8 @@ __all__ = [
     "SequenceData",
     "TIME_TOLERANCE",
     "apply_measurement_errors",
+    "compute_noise_from_lindbladians",
+    "find_root_brents",
     "get_max_rss_cpu",
     "get_max_rss_gpu",
     "get_target_times",
@@ -37,6 +42,7 @@ __all__ = [
     "interaction_matrix_from_positions",
     "is_evaluation_time",
     "krylov_exp",
+    "matmul_2x2_with_batched",
     "sequence_data_from_samples",
     "unix_like",
 ]

@@ -50,7 +50,8 @@ class DenseState:
         categories = self.data.shape[0]
         if categories <= SAMPLE_MAX_CATEGORIES or not self.data.is_cuda:
             probabilities = torch.abs(self.data.diagonal()) if density else torch.abs(self.data) ** 2
-            outcomes = sample_indices(probabilities, num_shots, SAMPLE_MAX_CATEGORIES).cpu()
+            # two arguments, as the reference calls it (its default limit is 2^24)
+            outcomes = sample_indices(probabilities, num_shots).cpu()
         else:
             from emulators_amd import _native as native
             data = self.data.contiguous()

@@ -502,3 +502,27 @@ def test_expm_column_vs_scipy(m, scale):
     ref = sl.expm(a)[:, 0]
     got = native.expm_column(a)
     assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+
+
+def test_emu_base_public_names():
+    """The names the reference's emu_base exports (emu_base/__init__.py:1-36)
+    are all importable from this package's emu_base."""
+    import emulators_amd.emu_base as eb
+    for name in ("get_max_rss_cpu", "get_max_rss_gpu", "compute_noise_from_lindbladians",
+                 "matmul_2x2_with_batched", "PulserData", "SequenceData", "is_evaluation_time",
+                 "find_root_brents", "krylov_exp", "HamiltonianType", "DEFAULT_MAX_KRYLOV_DIM",
+                 "DEVICE_COUNT", "apply_measurement_errors", "unix_like", "init_logging"):
+        assert hasattr(eb, name), name
+        assert name in eb.__all__, name
+
+
+def test_matmul_2x2_with_batched():
+    """test/emu_base/math/test_matmul.py:5-9: the batched product left @ right;
+    plus the shape check."""
+    from emulators_amd.emu_base import matmul_2x2_with_batched
+    g = torch.Generator().manual_seed(9)
+    left = torch.randn(2, 2, dtype=torch.complex128, generator=g)
+    right = torch.randn(10, 2, 5, dtype=torch.complex128, generator=g)
+    assert torch.allclose(left @ right, matmul_2x2_with_batched(left, right))
+    with pytest.raises(ValueError):
+        matmul_2x2_with_batched(torch.eye(3, dtype=torch.complex128), right)
