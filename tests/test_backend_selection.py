@@ -95,3 +95,58 @@ def test_max_krylov_dim_reaches_every_krylov_call(gpu_device, monkeypatch, solve
     assert type(impl.stepper) is stepper
     impl.step(0)
     assert dims and dims == [MAX_DIM] * len(dims)
+
+
+@pytest.mark.gpu
+@pytest.mark.usefixtures("restore_rng_state")
+def test_max_krylov_dim_reaches_backward(gpu_device, monkeypatch):
+    """The reference's test_max_krylov_dim_reaches_backward (test/emu_sv/
+    test_sv_backend.py:274-307): the per-step gradient uses the stepper's
+    max_krylov_dim in the forward krylov_exp, the backward krylov_exp of the
+    input state and the double_krylov of the pulse parameters."""
+    te_torch = importlib.import_module("emulators_amd.emu_sv.time_evolution.torch")
+    calls = {"krylov_exp": [], "double_krylov": []}
+    real_exp, real_double = te_torch.krylov_exp, te_torch.double_krylov
+    sig_exp, sig_double = inspect.signature(real_exp), inspect.signature(real_double)
+
+    def spy_exp(*args, **kwargs):
+        b = sig_exp.bind(*args, **kwargs)
+        b.apply_defaults()
+        calls["krylov_exp"].append(b.arguments["max_krylov_dim"])
+        return real_exp(*args, **kwargs)
+
+    def spy_double(*args, **kwargs):
+        b = sig_double.bind(*args, **kwargs)
+        calls["double_krylov"].append(b.arguments["max_krylov_dim"])
+        return real_double(*args, **kwargs)
+
+    monkeypatch.setattr(te_torch, "krylov_exp", spy_exp)
+    monkeypatch.setattr(te_torch, "double_krylov", spy_double)
+    torch.manual_seed(1337)
+    n = 2
+    om, de, ph = (torch.rand(n, dtype=torch.float64, requires_grad=True) for _ in range(3))
+    u = torch.triu(torch.rand(n, n, dtype=torch.float64), 1)
+    u = (u + u.T).requires_grad_(True)
+    psi = torch.randn(2**n, dtype=torch.complex128)
+    psi = (psi / psi.norm()).to(gpu_device).requires_grad_(True)
+    out, _ = EvolveStateVector(MAX_DIM).apply(1.0, om, de, ph, u, psi, 1e-10, None)
+    torch.autograd.grad(out.real.sum(), (om, de, ph, u, psi))
+    assert calls["krylov_exp"] == [MAX_DIM] * 2  # forward, then the input state's gradient
+    assert calls["double_krylov"] == [MAX_DIM]  # the pulse parameters' gradient
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("state_requires_grad", [True, False])
+def test_forward_output_requires_grad(gpu_device, state_requires_grad):
+    """The reference's test_forward_with_requires_grad (test/emu_sv/
+    test_time_evolution.py:47-55): the evolved state takes part in autograd
+    when either the state or the Hamiltonian parameters require grad."""
+    g = torch.Generator().manual_seed(4)
+    om, de, ph = (torch.rand(1, dtype=torch.float64, generator=g) for _ in range(3))
+    if not state_requires_grad:
+        om.requires_grad_(True)
+    u = torch.zeros(1, 1, dtype=torch.float64)
+    psi = torch.randn(2, dtype=torch.complex128, generator=g)
+    psi = (psi / psi.norm()).to(gpu_device).requires_grad_(state_requires_grad)
+    out, _ = EvolveStateVector().apply(0.3, om, de, ph, u, psi, 1e-5, None)
+    assert out.requires_grad
