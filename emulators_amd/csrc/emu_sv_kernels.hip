@@ -3133,6 +3133,21 @@ int static_scratch(hipStream_t st, void **out) {
 }
 }  // namespace
 
+namespace {
+// The device address of pinned (page-locked) host memory, or nullptr for
+// any other pointer (pageable memory keeps explicit copies).
+double *device_view_of_pinned(double *host) {
+  hipPointerAttribute_t attr{};
+  if (hipPointerGetAttributes(&attr, host) != hipSuccess) {
+    (void)hipGetLastError();  // a pageable pointer is not an error here
+    return nullptr;
+  }
+  if (attr.type != hipMemoryTypeHost || !attr.devicePointer || !attr.hostPointer) return nullptr;
+  const ptrdiff_t off = reinterpret_cast<char *>(host) - static_cast<char *>(attr.hostPointer);
+  return reinterpret_cast<double *>(static_cast<char *>(attr.devicePointer) + off);
+}
+}  // namespace
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -3868,9 +3883,17 @@ int emu_sv_krylov_step(int n, int dtype, const double *ham, double sigma_re, dou
   EventPair ev;
   for (hipEvent_t &x : ev.e)
     if (int rc = hip_check(hipEventCreateWithFlags(&x, hipEventDisableTiming), "krylov_step: event")) return rc;
+  // Pinned host records take the iterations' reductions straight from the
+  // kernel that forms them, so no copy command sits between two iterations
+  // (at 12-20 qubits an iteration is 40-80 us of GPU work, and each copy
+  // and the launch gap behind it cost about 10 us).  Pageable records keep
+  // the asynchronous copy.
+  double *host_rec_dev = device_view_of_pinned(host_records);
   int enqueued = 0, matvecs = 1;
   auto enqueue = [&](int j) -> int {
     double *rec = records + (size_t)j * kStepRow;
+    // where the reductions land: the host row itself, or the device row
+    double *out = host_rec_dev ? host_rec_dev + (size_t)j * kStepRow : rec;
     // iteration j reads w_j (row j + 1), orthogonalises it against basis
     // vectors j - 1 and j and writes r_j = w_{j+1} into row j + 2
     const void *pa = j > 0 ? (j == 1 ? v : row_ptr(j - 1)) : nullptr;
@@ -3879,15 +3902,16 @@ int emu_sv_krylov_step(int n, int dtype, const double *ham, double sigma_re, dou
     // workspace and stream: the iterations skip its stamp check
     t_diag_known_fresh = true;
     const int irc = emu_sv_lanczos_iterate_dev(n, dtype, ham, sigma_re, sigma_im, row_ptr(j + 1), pa, pb,
-                                               rec + 16, row_ptr(j + 2), rec, rec + 5,
-                                               records + (size_t)(j + 1) * kStepRow + 16, rec + 24,
+                                               rec + 16, row_ptr(j + 2), out, out + 5,
+                                               records + (size_t)(j + 1) * kStepRow + 16, out + 24,
                                                workspace, tile_log, min_contig,
                                                base_flags | EMU_SV_FLAG_KEEP_DIAG, stream);
     t_diag_known_fresh = false;
     if (irc) return irc;
-    if (int rc = hip_check(hipMemcpyAsync(host_records + (size_t)j * kStepRow, rec, kStepRow * sizeof(double),
-                                          hipMemcpyDeviceToHost, stream), "krylov_step: copy"))
-      return rc;
+    if (!host_rec_dev)
+      if (int rc = hip_check(hipMemcpyAsync(host_records + (size_t)j * kStepRow, rec, kStepRow * sizeof(double),
+                                            hipMemcpyDeviceToHost, stream), "krylov_step: copy"))
+        return rc;
     if (int rc = hip_check(hipEventRecord(ev.e[j & 1], stream), "krylov_step: event")) return rc;
     enqueued = j + 1;
     ++matvecs;

@@ -300,7 +300,10 @@ int emu_sv_arnoldi_orthonormalize(int n, int dtype, int count, const void *const
  *   v          device input state (2^n elements, not modified; out may alias it)
  *   basis      device buffer of max_dim + 1 vectors of 2^n elements
  *   records    device, emu_sv_krylov_step_records_doubles(max_dim) doubles
- *   host_records  pinned host buffer of the same size
+ *   host_records  pinned host buffer of the same size; the reduction that
+ *                 ends each iteration writes the iteration's record into it
+ *                 directly (pageable memory also works, through a copy per
+ *                 iteration)
  *   flags      kernel kind (tile, btile or LDS; n must cover its tiles),
  *              EMU_SV_FLAG_COMPLEX
  *   result_out host int[4]: iterations, converged, happy breakdown, matvecs
