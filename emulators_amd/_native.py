@@ -205,6 +205,15 @@ def _require_doubles(t: torch.Tensor, count: int, what: str) -> None:
         raise ValueError(f"{what}: needs a contiguous float64 device tensor of >= {count} elements")
 
 
+def _require_out_doubles(t: torch.Tensor, count: int, what: str) -> None:
+    """An output the kernels write: a contiguous float64 device tensor, or one
+    in pinned host memory (written there directly by the kernels)."""
+    if not (t.is_cuda or t.is_pinned()) or t.dtype != torch.float64 or not t.is_contiguous() \
+            or t.numel() < count:
+        raise ValueError(f"{what}: needs a contiguous float64 device or pinned host tensor of "
+                         f">= {count} elements")
+
+
 def _require_iteration(n: int, ham_dev: torch.Tensor, w: torch.Tensor, pa: torch.Tensor | None,
                        pb: torch.Tensor, r: torch.Tensor, coef: torch.Tensor, rec: torch.Tensor,
                        what: str) -> None:
@@ -215,7 +224,7 @@ def _require_iteration(n: int, ham_dev: torch.Tensor, w: torch.Tensor, pa: torch
         if t.dtype != w.dtype or t.numel() != (1 << n):
             raise ValueError(f"{what}: vectors must be 2**n elements of one dtype")
     _require_doubles(coef, 6, what + " (coefficients)")
-    _require_doubles(rec, 12, what + " (reductions)")
+    _require_out_doubles(rec, 12, what + " (reductions)")
 
 
 def pack_hamiltonian(omega, delta, phi, interaction) -> tuple[np.ndarray, bool]:
@@ -376,7 +385,7 @@ def lanczos_iterate_dev(n: int, ham_dev: torch.Tensor, sigma: complex, w: torch.
     coefficients (nxt, 6 doubles) and tout (n2, b, a, ww)."""
     _require_iteration(n, ham_dev, w, pa, pb, r, coef, rec, "lanczos_iterate_dev")
     _require_doubles(nxt, 6, "lanczos_iterate_dev (next coefficients)")
-    _require_doubles(tout, 6, "lanczos_iterate_dev (side outputs)")
+    _require_out_doubles(tout, 6, "lanczos_iterate_dev (side outputs)")
     flags = (EMU_SV_FLAG_COMPLEX if complex_coef else 0) | kernel_flags(kernel, partners, first_log)
     if keep_diag:
         flags |= EMU_SV_FLAG_KEEP_DIAG

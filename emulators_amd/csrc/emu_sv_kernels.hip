@@ -3134,6 +3134,11 @@ int static_scratch(hipStream_t st, void **out) {
 }  // namespace
 
 namespace {
+int lanczos_iterate_impl(int n, int dtype, const double *ham, double sigma_re, double sigma_im, void *w,
+                         const void *pa, const void *pb, const double *coef, void *r, double *norm2_out,
+                         double *dots_out, double *next_coef, double *t_out, void *workspace,
+                         int tile_log, int min_contig, int flags, hipStream_t stream);
+
 // The device address of pinned (page-locked) host memory, or nullptr for
 // any other pointer (pageable memory keeps explicit copies).
 double *device_view_of_pinned(double *host) {
@@ -3433,6 +3438,23 @@ int emu_sv_lanczos_iterate_dev(int n, int dtype, const double *ham, double sigma
                                double *norm2_out, double *dots_out, double *next_coef,
                                double *t_out, void *workspace, int tile_log, int min_contig,
                                int flags, hipStream_t stream) {
+  // the reductions and side outputs may sit in pinned host memory: the
+  // kernels then write them there (no copy command behind the iteration)
+  auto at = [](double *p) {
+    double *d = p ? device_view_of_pinned(p) : nullptr;
+    return d ? d : p;
+  };
+  return lanczos_iterate_impl(n, dtype, ham, sigma_re, sigma_im, w, pa, pb, coef, r, at(norm2_out),
+                              at(dots_out), next_coef, at(t_out), workspace, tile_log, min_contig, flags,
+                              stream);
+}
+}  // extern "C"
+
+namespace {
+int lanczos_iterate_impl(int n, int dtype, const double *ham, double sigma_re, double sigma_im, void *w,
+                         const void *pa, const void *pb, const double *coef, void *r, double *norm2_out,
+                         double *dots_out, double *next_coef, double *t_out, void *workspace,
+                         int tile_log, int min_contig, int flags, hipStream_t stream) {
   if (!next_coef || !t_out) return fail(EMU_SV_EINVAL, "lanczos_iterate_dev: null pointer");
   if (next_coef == coef) return fail(EMU_SV_EINVAL, "lanczos_iterate_dev: next_coef must not alias coef");
   NextCoefOut nc;
@@ -3444,6 +3466,9 @@ int emu_sv_lanczos_iterate_dev(int n, int dtype, const double *ham, double sigma
   return lanczos_matvec_dev_impl(n, dtype, ham, sigma_re, sigma_im, w, pa, pb, coef, r, norm2_out,
                                  dots_out, workspace, tile_log, min_contig, flags, stream, nc);
 }
+}  // namespace
+
+extern "C" {
 
 int emu_sv_lanczos_next_coef(const double *norm2, const double *dots, double sigma_re, double sigma_im,
                              const double *prev, double *next, double *t_out, hipStream_t stream) {
@@ -3901,7 +3926,7 @@ int emu_sv_krylov_step(int n, int dtype, const double *ham, double sigma_re, dou
     // the opening H.psi built the first-pass diagonal table on this
     // workspace and stream: the iterations skip its stamp check
     t_diag_known_fresh = true;
-    const int irc = emu_sv_lanczos_iterate_dev(n, dtype, ham, sigma_re, sigma_im, row_ptr(j + 1), pa, pb,
+    const int irc = lanczos_iterate_impl(n, dtype, ham, sigma_re, sigma_im, row_ptr(j + 1), pa, pb,
                                                rec + 16, row_ptr(j + 2), out, out + 5,
                                                records + (size_t)(j + 1) * kStepRow + 16, out + 24,
                                                workspace, tile_log, min_contig,

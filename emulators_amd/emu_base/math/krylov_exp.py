@@ -247,10 +247,10 @@ def _fused_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: float,
 
 
 class _PipelineBuffers:
-    """Per-(device, stream) records of the pipelined engine: for iteration j, device
-    row j holds [0:12] the reductions, [16:22] the coefficients it reads,
-    [24:30] the coefficients' side outputs (n2, b, a, ww of j+1); a pinned
-    host mirror receives rows as they complete."""
+    """Per-(device, stream) records of the pipelined engine: for iteration j,
+    device row j holds [16:22] the coefficients it reads; the pinned host row
+    j receives, from the iteration's last kernel, [0:12] the reductions and
+    [24:30] the coefficients' side outputs (n2, b, a, ww of j+1)."""
 
     _cache: dict = {}
     WIDTH = 32
@@ -333,10 +333,12 @@ def _pipelined_lanczos(op: ScaledHamiltonian, v: torch.Tensor, exp_tolerance: fl
         w = wbuf_of(j)
         # the first-pass diagonal table of the opening hpsi call serves every
         # iteration; the last reduction also forms iteration j+1's coefficients
+        # the reductions and side outputs go straight into the pinned host
+        # rows (no copy command between iterations); the coefficients the
+        # next iteration reads stay on the device
         native.lanczos_iterate_dev(n, params, sig, w, basis_at(j - 1) if j > 0 else None, basis_at(j),
-                                   dev[j, 16:22], r, dev[j, 0:12], dev[j + 1, 16:22], dev[j, 24:30],
+                                   dev[j, 16:22], r, host[j, 0:12], dev[j + 1, 16:22], host[j, 24:30],
                                    complex_coef=ham.complex, keep_diag=True, **kw)
-        host[j].copy_(dev[j], non_blocking=True)
         ev = torch.cuda.Event()
         ev.record(stream)
         events.append(ev)

@@ -216,7 +216,10 @@ int emu_sv_lanczos_matvec_dev(int n, int dtype, const double *ham, double sigma_
  * sequence: the reduction that ends the iteration also writes next_coef (6
  * doubles, device; must not alias coef) and t_out (6 doubles) exactly as
  * emu_sv_lanczos_next_coef(norm2_out, dots_out, sigma, coef, next_coef,
- * t_out) would.  Replaces the same reference code as the two calls. */
+ * t_out) would.  norm2_out, dots_out and t_out may be device memory or
+ * pinned host memory, which that kernel then writes directly (so a caller
+ * one iteration behind needs no copy command between iterations).
+ * Replaces the same reference code as the two calls. */
 int emu_sv_lanczos_iterate_dev(int n, int dtype, const double *ham, double sigma_re,
                                double sigma_im, void *w, const void *pa, const void *pb,
                                const double *coef, void *r, double *norm2_out,
